@@ -1,0 +1,218 @@
+"""ctypes binding of the C ABI in libffamd.so (include/fastfermion_amd.h).
+
+This is the array-level entry to the device engine used by bench.py and the
+parity tests; the object-level drop-in API (PauliString, ROT, propagate, ...)
+lives in the compiled module fastfermion_amd._ffcore, which calls the same ABI.
+There is no CPU fallback: every call raises when no HIP device is usable.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+import threading
+
+import numpy as np
+
+from ._abi import FP64, MAJORANA_GATE_DTYPE, PAULI_GATE_DTYPE
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libffamd.so")
+
+STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP error", -4: "no HIP device",
+          -5: "capacity exceeded"}
+
+EXPORTED = ("ffx_ctx_create", "ffx_ctx_destroy", "ffx_last_error", "ffx_pauli_propagate",
+            "ffx_majorana_propagate", "ffx_expect_last", "ffx_terms_free", "ffx_profile_get",
+            "ffx_device_count")
+
+
+class Options(ctypes.Structure):
+    _fields_ = [("maxdegree", ctypes.c_int32), ("topk", ctypes.c_int32),
+                ("max_xweight", ctypes.c_int32), ("xtrunc_period", ctypes.c_int32),
+                ("batched", ctypes.c_int32), ("precision", ctypes.c_int32),
+                ("mincoeff", ctypes.c_double)]
+
+
+class Terms(ctypes.Structure):
+    _fields_ = [("n_terms", ctypes.c_int64), ("n_words", ctypes.c_int32), ("reserved", ctypes.c_int32),
+                ("x", ctypes.POINTER(ctypes.c_uint64)), ("z", ctypes.POINTER(ctypes.c_uint64)),
+                ("c_re", ctypes.POINTER(ctypes.c_double)), ("c_im", ctypes.POINTER(ctypes.c_double))]
+
+
+class Profile(ctypes.Structure):
+    _fields_ = [("n_gates", ctypes.c_int64), ("updates", ctypes.c_int64), ("peak_terms", ctypes.c_int64),
+                ("launches", ctypes.c_int64), ("seconds_total", ctypes.c_double),
+                ("seconds_upload", ctypes.c_double), ("seconds_propagate", ctypes.c_double),
+                ("seconds_download", ctypes.c_double)]
+
+
+_lib = None
+_lib_lock = threading.Lock()
+
+
+def load():
+    """Load libffamd.so (raises OSError if it was not built)."""
+    global _lib
+    with _lib_lock:
+        if _lib is None:
+            if not os.path.exists(LIB_PATH):
+                raise OSError(f"{LIB_PATH} is missing: run __graft_entry__.build() first")
+            L = ctypes.CDLL(LIB_PATH)
+            vp, i32, i64, dbl = ctypes.c_void_p, ctypes.c_int32, ctypes.c_int64, ctypes.c_double
+            L.ffx_ctx_create.argtypes = [ctypes.c_int, ctypes.POINTER(vp)]
+            L.ffx_ctx_destroy.argtypes = [vp]
+            L.ffx_last_error.argtypes = [vp]
+            L.ffx_last_error.restype = ctypes.c_char_p
+            L.ffx_pauli_propagate.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i64, ctypes.POINTER(Options),
+                                              ctypes.POINTER(Terms), vp, vp]
+            L.ffx_majorana_propagate.argtypes = [vp, vp, i64, i32, vp, vp, vp, i64, i32, dbl, i32,
+                                                 ctypes.POINTER(Terms), vp, vp]
+            L.ffx_expect_last.argtypes = [vp, vp, vp]
+            L.ffx_terms_free.argtypes = [ctypes.POINTER(Terms)]
+            L.ffx_profile_get.argtypes = [vp, ctypes.POINTER(Profile)]
+            L.ffx_device_count.argtypes = []
+            _lib = L
+    return _lib
+
+
+def device_count() -> int:
+    return int(load().ffx_device_count())
+
+
+def _p(a):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+class Result:
+    """Propagated observable: term-major uint64 keys and complex coefficients."""
+
+    def __init__(self, x, z, c):
+        self.x, self.z, self.c = x, z, c
+
+    def __len__(self):
+        return len(self.c)
+
+
+class Context:
+    """One HIP device context (stream + device memory pools) of libffamd.so."""
+
+    def __init__(self, device: int = 0):
+        self._L = load()
+        h = ctypes.c_void_p()
+        rc = self._L.ffx_ctx_create(device, ctypes.byref(h))
+        if rc != 0:
+            raise RuntimeError(f"fastfermion_amd: cannot create a HIP context on device {device} "
+                               f"({STATUS.get(rc, rc)})")
+        self._h = h
+
+    def close(self):
+        if getattr(self, "_h", None):
+            self._L.ffx_ctx_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, rc):
+        if rc != 0:
+            msg = self._L.ffx_last_error(self._h).decode(errors="replace")
+            if rc == -2:
+                raise MemoryError(msg)
+            raise RuntimeError(f"fastfermion_amd: {STATUS.get(rc, rc)}: {msg}")
+
+    @staticmethod
+    def _collect(t: Terms, pauli: bool) -> Result:
+        n, W = t.n_terms, t.n_words
+        if n == 0:
+            e = np.zeros((0, W), np.uint64)
+            return Result(e, e.copy() if pauli else None, np.zeros(0, np.complex128))
+        x = np.ctypeslib.as_array(t.x, shape=(n * W,)).reshape(n, W).copy()
+        z = np.ctypeslib.as_array(t.z, shape=(n * W,)).reshape(n, W).copy() if pauli else None
+        re = np.ctypeslib.as_array(t.c_re, shape=(n,)).copy()
+        im = np.ctypeslib.as_array(t.c_im, shape=(n,)).copy()
+        return Result(x, z, re + 1j * im)
+
+    def pauli_propagate(self, gates, n_words, x, z, c, maxdegree=-1, mincoeff=0.0, topk=0, max_xweight=-1,
+                        xtrunc_period=1, batched=True, precision=FP64, download=True, state=None,
+                        want_expect=False):
+        gates = np.ascontiguousarray(gates, dtype=PAULI_GATE_DTYPE)
+        x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, n_words)
+        z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, n_words)
+        c = np.asarray(c, dtype=np.complex128).reshape(-1)
+        re = np.ascontiguousarray(c.real)
+        im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+        opts = Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        t = Terms()
+        ex = np.zeros(2)
+        st = None
+        if state is not None:
+            st = np.zeros(4, np.uint64)
+            st[: len(state)] = state
+        rc = self._L.ffx_pauli_propagate(self._h, _p(gates), len(gates), n_words, _p(x), _p(z), _p(re), _p(im),
+                                         len(c), ctypes.byref(opts), ctypes.byref(t) if download else None,
+                                         _p(st), _p(ex) if (want_expect or state is not None) else None)
+        self._check(rc)
+        res = None
+        if download:
+            try:
+                res = self._collect(t, True)
+            finally:
+                self._L.ffx_terms_free(ctypes.byref(t))
+        if want_expect or state is not None:
+            return res, complex(ex[0], ex[1])
+        return res
+
+    def majorana_propagate(self, gates, n_words, m, c, maxdegree=-1, mincoeff=0.0, precision=FP64,
+                           download=True, occ=None, want_expect=False):
+        gates = np.ascontiguousarray(gates, dtype=MAJORANA_GATE_DTYPE)
+        m = np.ascontiguousarray(m, dtype=np.uint64).reshape(-1, n_words)
+        c = np.asarray(c, dtype=np.complex128).reshape(-1)
+        re = np.ascontiguousarray(c.real)
+        im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+        t = Terms()
+        ex = np.zeros(2)
+        oc = None
+        if occ is not None:
+            oc = np.zeros(4, np.uint64)
+            oc[: len(occ)] = occ
+        rc = self._L.ffx_majorana_propagate(self._h, _p(gates), len(gates), n_words, _p(m), _p(re), _p(im),
+                                            len(c), maxdegree, mincoeff, precision,
+                                            ctypes.byref(t) if download else None, _p(oc),
+                                            _p(ex) if (want_expect or occ is not None) else None)
+        self._check(rc)
+        res = None
+        if download:
+            try:
+                res = self._collect(t, False)
+            finally:
+                self._L.ffx_terms_free(ctypes.byref(t))
+        if want_expect or occ is not None:
+            return res, complex(ex[0], ex[1])
+        return res
+
+    def expect_last(self, state=None) -> complex:
+        st = np.zeros(4, np.uint64)
+        if state is not None:
+            st[: len(state)] = state
+        ex = np.zeros(2)
+        self._check(self._L.ffx_expect_last(self._h, _p(st), _p(ex)))
+        return complex(ex[0], ex[1])
+
+    def profile(self) -> dict:
+        p = Profile()
+        self._check(self._L.ffx_profile_get(self._h, ctypes.byref(p)))
+        return {k: getattr(p, k) for k, _ in Profile._fields_}
+
+
+_default = None
+
+
+def default_context() -> Context:
+    """Process-wide context on device 0 (created on first use)."""
+    global _default
+    if _default is None:
+        _default = Context(0)
+    return _default

@@ -1,0 +1,865 @@
+// ffx_lib.hip -- host engine and C ABI of libffamd.so (declared in
+// include/fastfermion_amd.h).
+//
+// The host side restates the control flow of the reference drivers --
+// reverse walk of the circuit, Clifford blocks, greedy commuting batches,
+// truncation points (src/pauli/propagate.h:297-366) and the per-gate threshold of
+// the Majorana driver (src/majorana_propagation.h:18-33) -- as a list of device
+// steps, then runs every step as HIP kernels on one stream.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <string>
+#include <type_traits>
+#include <utility>
+#include <vector>
+
+#include "../../include/fastfermion_amd.h"
+#include "ffx_kernels.h"
+
+using namespace ffx;
+
+struct ffx_ctx {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+    // term store
+    uint64_t* key = nullptr;
+    uint64_t* key2 = nullptr;
+    void* coef = nullptr;
+    void* coef2 = nullptr;
+    uint64_t* index = nullptr;
+    size_t cap = 0;      // slots
+    size_t hslots = 0;   // index entries (power of two, >= 2*cap)
+    size_t key_planes = 0, coef_planes = 0, coef_elem = 0;  // layout of the current store
+    Ctrl* ctrl = nullptr;
+    Ctrl* ctrl_host = nullptr;  // pinned
+    uint32_t* counts = nullptr;
+    size_t counts_cap = 0;
+    uint32_t* total = nullptr;
+    double* partial = nullptr;
+    double* result = nullptr;
+    double* result_host = nullptr;  // pinned, 2 doubles
+    void* staging = nullptr;
+    size_t staging_bytes = 0;
+    CliffGate* cliff = nullptr;
+    size_t cliff_cap = 0;
+    // description of the device-resident result
+    int last_kind = 0;  // 0 none, 1 Pauli, 2 Majorana
+    int last_W = 0, last_K = 0, last_prec = 0;
+    ffx_profile prof{};
+    int64_t launches = 0;
+};
+
+namespace {
+
+#define HIPCHK(expr)                                                                                   \
+    do {                                                                                               \
+        hipError_t _e = (expr);                                                                        \
+        if (_e != hipSuccess) {                                                                        \
+            ctx->err = std::string("HIP error: ") + hipGetErrorString(_e) + " at " #expr;              \
+            return _e == hipErrorOutOfMemory ? FFX_ENOMEM : FFX_EHIP;                                  \
+        }                                                                                              \
+    } while (0)
+
+#define CHK(expr)                   \
+    do {                            \
+        int _r = (expr);            \
+        if (_r != FFX_OK) return _r; \
+    } while (0)
+
+inline double now_s() {
+    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
+}
+
+int set_err(ffx_ctx* ctx, int code, const std::string& msg) {
+    ctx->err = msg;
+    return code;
+}
+
+size_t pow2ceil(size_t v) {
+    size_t p = 1;
+    while (p < v) p <<= 1;
+    return p;
+}
+
+template <class T>
+int dev_alloc(ffx_ctx* ctx, T** p, size_t bytes) {
+    *p = nullptr;
+    if (bytes == 0) bytes = 256;
+    hipError_t e = hipMalloc((void**)p, bytes);
+    if (e != hipSuccess) {
+        *p = nullptr;
+        (void)hipGetLastError();
+        return set_err(ctx, FFX_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
+    }
+    return FFX_OK;
+}
+
+template <class T>
+void dev_free(T*& p) {
+    if (p) (void)hipFree((void*)p);
+    p = nullptr;
+}
+
+void free_store(ffx_ctx* ctx) {
+    dev_free(ctx->key);
+    dev_free(ctx->key2);
+    dev_free(ctx->coef);
+    dev_free(ctx->coef2);
+    dev_free(ctx->index);
+    ctx->cap = ctx->hslots = 0;
+}
+
+unsigned grid_for(size_t n, size_t per_block, unsigned max_grid = 65535) {
+    size_t g = (n + per_block - 1) / per_block;
+    if (g < 1) g = 1;
+    if (g > max_grid) g = max_grid;
+    return (unsigned)g;
+}
+
+// One device step of a propagation.
+struct Step {
+    int kind = 0;        // 0 rotation, 1 Clifford block
+    int64_t gate = 0;    // rotation: gate index
+    int trunc = 0;       // rotation: truncate_all runs after it
+    int rc = 0;          // rotation: rot_count argument of that truncate_all
+    int64_t cb = 0, ce = 0;  // Clifford block [cb, ce)
+};
+
+template <int W>
+bool rot_commute(const ffx_pauli_gate& a, const ffx_pauli_gate& b) {
+    int p1 = 0, p2 = 0;
+    for (int w = 0; w < W; ++w) {
+        p1 += __builtin_popcountll(a.x[w] & b.z[w]);
+        p2 += __builtin_popcountll(a.z[w] & b.x[w]);
+    }
+    return (p1 & 1) == (p2 & 1);
+}
+
+// Schedule of src/pauli/propagate.h:318-366 (serial driver): the circuit is
+// walked from the last gate to the first; runs of Clifford gates form blocks;
+// rotations either truncate after every gate (batched = false) or are grouped by
+// batch_commuting_gates (propagate.h:272-291) and truncate once per batch.
+template <int W>
+std::vector<Step> pauli_schedule(const ffx_pauli_gate* g, int64_t n, bool batched) {
+    std::vector<Step> steps;
+    int rot_count = 0;
+    bool pending = false;
+    int64_t cend = 0;
+    std::vector<int64_t> buf;
+    auto flush = [&]() {
+        if (buf.empty()) return;
+        size_t s = 0;
+        while (s < buf.size()) {
+            size_t e = s + 1;
+            while (e < buf.size()) {
+                bool ok = true;
+                for (size_t t = s; t < e; ++t)
+                    if (!rot_commute<W>(g[buf[e]], g[buf[t]])) {
+                        ok = false;
+                        break;
+                    }
+                if (!ok) break;
+                ++e;
+            }
+            for (size_t t = s; t < e; ++t) {
+                Step st;
+                st.kind = 0;
+                st.gate = buf[t];
+                steps.push_back(st);
+            }
+            rot_count += (int)(e - s);
+            steps.back().trunc = 1;
+            steps.back().rc = rot_count - 1;
+            s = e;
+        }
+        buf.clear();
+    };
+    for (int64_t i = n - 1; i >= 0; --i) {
+        if (g[i].kind != FFX_ROT) {
+            if (batched) flush();
+            if (!pending) {
+                cend = i;
+                pending = true;
+            }
+        } else {
+            if (pending) {
+                Step c;
+                c.kind = 1;
+                c.cb = i + 1;
+                c.ce = cend + 1;
+                steps.push_back(c);
+                pending = false;
+            }
+            if (batched)
+                buf.push_back(i);
+            else {
+                Step st;
+                st.kind = 0;
+                st.gate = i;
+                st.trunc = 1;
+                st.rc = rot_count;
+                steps.push_back(st);
+                rot_count++;
+            }
+        }
+    }
+    if (batched) flush();
+    if (pending) {
+        Step c;
+        c.kind = 1;
+        c.cb = 0;
+        c.ce = cend + 1;
+        steps.push_back(c);
+    }
+    return steps;
+}
+
+template <class CP>
+typename CP::fac_t make_fac(double v);
+template <>
+F64::fac_t make_fac<F64>(double v) {
+    return v;
+}
+template <>
+F32::fac_t make_fac<F32>(double v) {
+    return (float)v;
+}
+template <>
+F32C::fac_t make_fac<F32C>(double v) {
+    float h = (float)v;
+    return F32C::fac_t{h, (float)(v - (double)h)};
+}
+
+// Engine over one term store; Alg = PauliAlg<W> or MajoranaAlg<W>.
+template <class Alg, class CP, int K>
+struct Engine {
+    static constexpr int NP = Alg::NP;
+    using store_t = typename CP::store_t;
+    ffx_ctx* ctx;
+    uint32_t n_host = 0;     // slots in use (exact after sync)
+    uint32_t dead_host = 0;  // erased slots (exact after sync)
+
+    explicit Engine(ffx_ctx* c) : ctx(c) {}
+
+    Store<CP, K> store() const {
+        Store<CP, K> s;
+        s.key = ctx->key;
+        s.coef = (store_t*)ctx->coef;
+        s.coef2 = (store_t*)ctx->coef2;
+        s.index = ctx->index;
+        s.hmask = ctx->hslots - 1;
+        s.cap = ctx->cap;
+        s.ctrl = ctx->ctrl;
+        return s;
+    }
+
+    size_t coef_planes() const { return (size_t)K * CP::NS; }
+
+    int launch_check() {
+        ctx->launches++;
+        hipError_t e = hipGetLastError();
+        if (e != hipSuccess) return set_err(ctx, FFX_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
+        return FFX_OK;
+    }
+
+    int rebuild_index() {
+        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx->hslots, kBlock, 8192)), dim3(kBlock), 0, ctx->stream,
+                           ctx->index, ctx->hslots, kEmpty);
+        CHK(launch_check());
+        hipLaunchKernelGGL(k_index_build<NP>, dim3(grid_for(n_host, kBlock, 16384)), dim3(kBlock), 0, ctx->stream,
+                           ctx->key, ctx->cap, ctx->index, ctx->hslots - 1, ctx->ctrl);
+        return launch_check();
+    }
+
+    // Make room for `need` slots; existing slots [0, n_host) are preserved.
+    int reserve(size_t need) {
+        if (need <= ctx->cap && ctx->key_planes == NP && ctx->coef_planes == coef_planes() &&
+            ctx->coef_elem == sizeof(store_t))
+            return FFX_OK;
+        if (need > 0x7FFFFFF0ull) return set_err(ctx, FFX_ECAPACITY, "term count exceeds 2^31 slots");
+        size_t ncap = pow2ceil(std::max<size_t>(need, 1024));
+        if (ncap < 2 * ctx->cap && ctx->key_planes == NP) ncap = std::max<size_t>(ncap, 2 * ctx->cap);
+        uint64_t *nkey = nullptr, *nkey2 = nullptr, *nindex = nullptr;
+        void *ncoef = nullptr, *ncoef2 = nullptr;
+        const size_t cbytes = coef_planes() * ncap * sizeof(store_t);
+        int rc = dev_alloc(ctx, &nkey, NP * ncap * sizeof(uint64_t));
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &nkey2, NP * ncap * sizeof(uint64_t));
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &ncoef, cbytes);
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &ncoef2, cbytes);
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &nindex, 2 * ncap * sizeof(uint64_t));
+        if (rc != FFX_OK) {
+            dev_free(nkey);
+            dev_free(nkey2);
+            dev_free(ncoef);
+            dev_free(ncoef2);
+            dev_free(nindex);
+            return rc;
+        }
+        if (n_host > 0 && ctx->key && ctx->key_planes == NP && ctx->coef_planes == coef_planes()) {
+            for (int q = 0; q < NP; ++q)
+                HIPCHK(hipMemcpyAsync(nkey + q * ncap, ctx->key + q * ctx->cap, n_host * sizeof(uint64_t),
+                                      hipMemcpyDeviceToDevice, ctx->stream));
+            for (size_t q = 0; q < coef_planes(); ++q)
+                HIPCHK(hipMemcpyAsync((store_t*)ncoef + q * ncap, (store_t*)ctx->coef + q * ctx->cap,
+                                      n_host * sizeof(store_t), hipMemcpyDeviceToDevice, ctx->stream));
+        }
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        free_store(ctx);
+        ctx->key = nkey;
+        ctx->key2 = nkey2;
+        ctx->coef = ncoef;
+        ctx->coef2 = ncoef2;
+        ctx->index = nindex;
+        ctx->cap = ncap;
+        ctx->hslots = 2 * ncap;
+        ctx->key_planes = NP;
+        ctx->coef_planes = coef_planes();
+        ctx->coef_elem = sizeof(store_t);
+        return rebuild_index();
+    }
+
+    int ensure_counts(size_t nb) {
+        if (nb + 1 <= ctx->counts_cap) return FFX_OK;
+        dev_free(ctx->counts);
+        size_t c = pow2ceil(nb + 1);
+        CHK(dev_alloc(ctx, &ctx->counts, c * sizeof(uint32_t)));
+        ctx->counts_cap = c;
+        return FFX_OK;
+    }
+
+    int sync_ctrl() {
+        HIPCHK(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        n_host = ctx->ctrl_host->n;
+        dead_host = ctx->ctrl_host->n_dead;
+        if (ctx->ctrl_host->overflow) return set_err(ctx, FFX_EHIP, "internal error: term store overflow");
+        return FFX_OK;
+    }
+
+    // Drop erased slots (order preserving) and rebuild the index.
+    int compact() {
+        if (n_host == 0) return FFX_OK;
+        const size_t nb = (n_host + kTile - 1) / kTile;
+        CHK(ensure_counts(nb));
+        Store<CP, K> s = store();
+        hipLaunchKernelGGL((k_count_live<CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, s, ctx->counts);
+        CHK(launch_check());
+        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->counts, (uint32_t)nb,
+                           ctx->total);
+        CHK(launch_check());
+        hipLaunchKernelGGL((k_compact<NP, CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, s, ctx->counts,
+                           ctx->key2, (store_t*)ctx->coef2);
+        CHK(launch_check());
+        uint32_t total = 0;
+        HIPCHK(hipMemcpyAsync(&total, ctx->total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        std::swap(ctx->key, ctx->key2);
+        std::swap(ctx->coef, ctx->coef2);
+        n_host = total;
+        dead_host = 0;
+        HIPCHK(hipMemcpyAsync(&ctx->ctrl->n, &n_host, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&ctx->ctrl->n_dead, &dead_host, sizeof(uint32_t), hipMemcpyHostToDevice,
+                              ctx->stream));
+        return rebuild_index();
+    }
+
+    int upload(const uint64_t* hx, const uint64_t* hz, const double* re, const double* im, int64_t n0) {
+        n_host = 0;
+        dead_host = 0;
+        CHK(reserve(std::max<size_t>(2 * (size_t)n0 + 64, 1024)));
+        Ctrl c{};
+        c.n = (uint32_t)n0;
+        HIPCHK(hipMemcpyAsync(ctx->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+        if (n0 > 0) {
+            constexpr int W = Alg::W;
+            const size_t kbytes = (size_t)n0 * W * sizeof(uint64_t);
+            const size_t need = (Alg::kPauli ? 2 : 1) * kbytes + 2 * (size_t)n0 * sizeof(double);
+            CHK(ensure_staging(need));
+            char* sp = (char*)ctx->staging;
+            uint64_t* dx = (uint64_t*)sp;
+            uint64_t* dz = Alg::kPauli ? (uint64_t*)(sp + kbytes) : nullptr;
+            double* dre = (double*)(sp + (Alg::kPauli ? 2 : 1) * kbytes);
+            double* dim = dre + n0;
+            HIPCHK(hipMemcpyAsync(dx, hx, kbytes, hipMemcpyHostToDevice, ctx->stream));
+            if (Alg::kPauli) HIPCHK(hipMemcpyAsync(dz, hz, kbytes, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(dre, re, n0 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            if (im)
+                HIPCHK(hipMemcpyAsync(dim, im, n0 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL((k_unpack<Alg, CP, K>), dim3(grid_for(n0, kBlock, 16384)), dim3(kBlock), 0,
+                               ctx->stream, store(), dx, dz, dre, im ? dim : nullptr, (uint32_t)n0);
+            CHK(launch_check());
+        }
+        n_host = (uint32_t)n0;
+        CHK(rebuild_index());
+        hipLaunchKernelGGL(k_index_check<NP>, dim3(grid_for(n_host, kBlock, 16384)), dim3(kBlock), 0, ctx->stream,
+                           ctx->key, ctx->cap, ctx->index, ctx->hslots - 1, ctx->ctrl);
+        CHK(launch_check());
+        CHK(sync_ctrl());
+        if (ctx->ctrl_host->dup) return set_err(ctx, FFX_EINVAL, "observable has duplicate keys");
+        return FFX_OK;
+    }
+
+    int ensure_staging(size_t bytes) {
+        if (bytes <= ctx->staging_bytes) return FFX_OK;
+        dev_free(ctx->staging);
+        ctx->staging_bytes = 0;
+        CHK(dev_alloc(ctx, &ctx->staging, bytes));
+        ctx->staging_bytes = bytes;
+        return FFX_OK;
+    }
+
+    int rotation(const typename Alg::Gate& g, const RotParams<CP>& rp) {
+        CHK(reserve(2 * (size_t)n_host + 64));
+        const unsigned grid = grid_for(n_host, kBlock, 16384);
+        hipLaunchKernelGGL((k_rot<Alg, CP, K>), dim3(grid), dim3(kBlock), 0, ctx->stream, store(), g, rp);
+        CHK(launch_check());
+        std::swap(ctx->coef, ctx->coef2);
+        return sync_ctrl();
+    }
+
+    int clifford(const ffx_pauli_gate* gates, int64_t cb, int64_t ce) {
+        if constexpr (Alg::kPauli) {
+            const size_t ng = (size_t)(ce - cb);
+            if (ng > ctx->cliff_cap) {
+                dev_free(ctx->cliff);
+                CHK(dev_alloc(ctx, &ctx->cliff, ng * sizeof(CliffGate)));
+                ctx->cliff_cap = ng;
+            }
+            std::vector<CliffGate> h(ng);
+            for (size_t t = 0; t < ng; ++t) h[t] = CliffGate{gates[cb + t].kind, gates[cb + t].q0, gates[cb + t].q1, 0};
+            HIPCHK(hipMemcpyAsync(ctx->cliff, h.data(), ng * sizeof(CliffGate), hipMemcpyHostToDevice, ctx->stream));
+            hipLaunchKernelGGL((k_clifford<Alg::W, CP, K>), dim3(grid_for(n_host, kBlock, 16384)), dim3(kBlock), 0,
+                               ctx->stream, store(), ctx->cliff, (int)ng);
+            CHK(launch_check());
+            CHK(rebuild_index());
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        }
+        return FFX_OK;
+    }
+
+    int expect(const uint64_t* state, double out[2]) {
+        StateWords sw{};
+        if (state)
+            for (int w = 0; w < Alg::W && w < FFX_MAX_WORDS; ++w) sw.s[w] = state[w];
+        const unsigned nb = grid_for(n_host, kBlock, 1024);
+        hipLaunchKernelGGL((k_expect_partial<Alg, CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, store(), sw,
+                           ctx->partial);
+        CHK(launch_check());
+        hipLaunchKernelGGL(k_expect_final, dim3(1), dim3(64), 0, ctx->stream, ctx->partial, (int)nb, ctx->result);
+        CHK(launch_check());
+        HIPCHK(hipMemcpyAsync(ctx->result_host, ctx->result, 2 * sizeof(double), hipMemcpyDeviceToHost,
+                              ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        out[0] = ctx->result_host[0];
+        out[1] = ctx->result_host[1];
+        return FFX_OK;
+    }
+
+    int download(ffx_terms* out) {
+        constexpr int W = Alg::W;
+        std::memset(out, 0, sizeof(*out));
+        out->n_words = W;
+        const size_t nb = (n_host + kTile - 1) / kTile;
+        uint32_t total = 0;
+        if (n_host > 0) {
+            CHK(ensure_counts(nb));
+            Store<CP, K> s = store();
+            hipLaunchKernelGGL((k_count_live<CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, s, ctx->counts);
+            CHK(launch_check());
+            hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->counts, (uint32_t)nb,
+                               ctx->total);
+            CHK(launch_check());
+            HIPCHK(hipMemcpyAsync(&total, ctx->total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+        }
+        const size_t nt = total;
+        const size_t kb = nt * W * sizeof(uint64_t);
+        out->x = (uint64_t*)malloc(std::max<size_t>(kb, 8));
+        out->z = Alg::kPauli ? (uint64_t*)malloc(std::max<size_t>(kb, 8)) : nullptr;
+        out->c_re = (double*)malloc(std::max<size_t>(nt * sizeof(double), 8));
+        out->c_im = (double*)malloc(std::max<size_t>(nt * sizeof(double), 8));
+        if (!out->x || (Alg::kPauli && !out->z) || !out->c_re || !out->c_im) {
+            ffx_terms_free(out);
+            return set_err(ctx, FFX_ENOMEM, "host allocation of the result failed");
+        }
+        out->n_terms = (int64_t)nt;
+        if (nt == 0) return FFX_OK;
+        const size_t need = (Alg::kPauli ? 2 : 1) * kb + 2 * nt * sizeof(double);
+        CHK(ensure_staging(need));
+        char* sp = (char*)ctx->staging;
+        uint64_t* ox = (uint64_t*)sp;
+        uint64_t* oz = Alg::kPauli ? (uint64_t*)(sp + kb) : nullptr;
+        double* ore = (double*)(sp + (Alg::kPauli ? 2 : 1) * kb);
+        double* oim = ore + nt;
+        hipLaunchKernelGGL((k_pack<Alg, CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, store(), ctx->counts, ox, oz,
+                           ore, oim);
+        CHK(launch_check());
+        HIPCHK(hipMemcpyAsync(out->x, ox, kb, hipMemcpyDeviceToHost, ctx->stream));
+        if (Alg::kPauli) HIPCHK(hipMemcpyAsync(out->z, oz, kb, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(out->c_re, ore, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(out->c_im, oim, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return FFX_OK;
+    }
+
+    void finish_profile() {
+        ctx->prof.n_gates = (int64_t)ctx->ctrl_host->n_rot;
+        ctx->prof.updates = (int64_t)ctx->ctrl_host->updates;
+        ctx->prof.peak_terms = (int64_t)ctx->ctrl_host->peak;
+    }
+};
+
+template <int W>
+typename PauliAlg<W>::Gate pauli_gate_dev(const ffx_pauli_gate& g) {
+    typename PauliAlg<W>::Gate d{};
+    int pxz = 0;
+    for (int w = 0; w < W; ++w) {
+        d.p[w] = g.x[w];
+        d.p[W + w] = g.z[w];
+        pxz += __builtin_popcountll(g.x[w] & g.z[w]);
+    }
+    d.pxz = pxz;
+    return d;
+}
+
+template <int W>
+typename MajoranaAlg<W>::Gate majorana_gate_dev(const ffx_majorana_gate& g) {
+    typename MajoranaAlg<W>::Gate d{};
+    int deg = 0;
+    for (int w = 0; w < W; ++w) {
+        d.p[w] = g.m[w];
+        deg += __builtin_popcountll(g.m[w]);
+    }
+    // suffix parity: bit t set iff an odd number of axis bits lie strictly above t
+    int par = 0;
+    for (int t = 64 * W - 1; t >= 0; --t) {
+        if (par) d.sm[t >> 6] |= 1ull << (t & 63);
+        if ((g.m[t >> 6] >> (t & 63)) & 1ull) par ^= 1;
+    }
+    d.deg = deg;
+    d.r = g.r;
+    return d;
+}
+
+template <int W, class CP, int K>
+int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const uint64_t* x, const uint64_t* z,
+              const double* re, const double* im, int64_t n_terms, const ffx_options& o, ffx_terms* out,
+              const uint64_t* state, double* out_expect) {
+    using Alg = PauliAlg<W>;
+    Engine<Alg, CP, K> eng(ctx);
+    const double t0 = now_s();
+    CHK(eng.upload(x, z, re, im, n_terms));
+    const double t1 = now_s();
+    const std::vector<Step> steps = pauli_schedule<W>(gates, n_gates, o.batched != 0);
+    const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
+    for (const Step& s : steps) {
+        if (s.kind == 0) {
+            const ffx_pauli_gate& g = gates[s.gate];
+            RotParams<CP> rp{};
+            rp.cs = make_fac<CP>(std::cos(g.theta));
+            rp.sn = make_fac<CP>(std::sin(g.theta));
+            rp.maxdeg = maxdeg;
+            rp.mincoeff = o.mincoeff;
+            rp.trunc_thresh = (s.trunc && o.mincoeff > 0) ? 1 : 0;
+            rp.trunc_xw = (s.trunc && o.max_xweight >= 0 && o.xtrunc_period > 0 &&
+                           (s.rc + 1) % o.xtrunc_period == 0 && o.topk <= 0)
+                              ? 1
+                              : 0;
+            rp.max_xw = o.max_xweight;
+            CHK(eng.rotation(pauli_gate_dev<W>(g), rp));
+            if (s.trunc && o.topk > 0)
+                return set_err(ctx, FFX_EINVAL, "topk truncation is not supported on the device path");
+            if (s.trunc && eng.dead_host > 0 && (size_t)eng.dead_host * 4 > eng.n_host) CHK(eng.compact());
+        } else {
+            CHK(eng.clifford(gates, s.cb, s.ce));
+        }
+    }
+    CHK(eng.sync_ctrl());
+    eng.finish_profile();
+    const double t2 = now_s();
+    if (out_expect) CHK(eng.expect(state, out_expect));
+    if (out) CHK(eng.download(out));
+    const double t3 = now_s();
+    ctx->prof.seconds_upload = t1 - t0;
+    ctx->prof.seconds_propagate = t2 - t1;
+    ctx->prof.seconds_download = t3 - t2;
+    ctx->prof.seconds_total = t3 - t0;
+    ctx->last_kind = 1;
+    ctx->last_W = W;
+    ctx->last_K = K;
+    ctx->last_prec = std::is_same<CP, F64>::value ? FFX_FP64 : (std::is_same<CP, F32>::value ? FFX_FP32 : FFX_FP32C);
+    return FFX_OK;
+}
+
+template <int W, class CP, int K>
+int run_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, const uint64_t* m, const double* re,
+                 const double* im, int64_t n_terms, int maxdegree, double mincoeff, ffx_terms* out,
+                 const uint64_t* occ, double* out_expect) {
+    using Alg = MajoranaAlg<W>;
+    Engine<Alg, CP, K> eng(ctx);
+    const double t0 = now_s();
+    CHK(eng.upload(m, nullptr, re, im, n_terms));
+    const double t1 = now_s();
+    const int maxdeg = maxdegree < 0 ? (1 << 30) : maxdegree;
+    for (int64_t i = n_gates - 1; i >= 0; --i) {
+        const ffx_majorana_gate& g = gates[i];
+        RotParams<CP> rp{};
+        rp.cs = make_fac<CP>(std::cos(g.theta));
+        rp.sn = make_fac<CP>(std::sin(g.theta));
+        rp.maxdeg = maxdeg;
+        rp.mincoeff = mincoeff;
+        rp.trunc_thresh = mincoeff > 0 ? 1 : 0;
+        CHK(eng.rotation(majorana_gate_dev<W>(g), rp));
+        if (eng.dead_host > 0 && (size_t)eng.dead_host * 4 > eng.n_host) CHK(eng.compact());
+    }
+    CHK(eng.sync_ctrl());
+    eng.finish_profile();
+    const double t2 = now_s();
+    if (out_expect) CHK(eng.expect(occ, out_expect));
+    if (out) CHK(eng.download(out));
+    const double t3 = now_s();
+    ctx->prof.seconds_upload = t1 - t0;
+    ctx->prof.seconds_propagate = t2 - t1;
+    ctx->prof.seconds_download = t3 - t2;
+    ctx->prof.seconds_total = t3 - t0;
+    ctx->last_kind = 2;
+    ctx->last_W = W;
+    ctx->last_K = K;
+    ctx->last_prec = std::is_same<CP, F64>::value ? FFX_FP64 : (std::is_same<CP, F32>::value ? FFX_FP32 : FFX_FP32C);
+    return FFX_OK;
+}
+
+template <class Alg, class CP, int K>
+int expect_last_impl(ffx_ctx* ctx, const uint64_t* state, double* out) {
+    Engine<Alg, CP, K> eng(ctx);
+    CHK(eng.sync_ctrl());
+    return eng.expect(state, out);
+}
+
+int mono_degree_mod4_r(const uint64_t* m, int W) {
+    int d = 0;
+    for (int w = 0; w < W; ++w) d += __builtin_popcountll(m[w]);
+    return (d >> 1) & 1;
+}
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// C ABI
+// ---------------------------------------------------------------------------
+extern "C" {
+
+int ffx_device_count(void) {
+    int n = 0;
+    if (hipGetDeviceCount(&n) != hipSuccess) {
+        (void)hipGetLastError();
+        return 0;
+    }
+    return n;
+}
+
+int ffx_ctx_create(int device, ffx_ctx** out) {
+    if (!out) return FFX_EINVAL;
+    *out = nullptr;
+    const int nd = ffx_device_count();
+    if (nd <= 0 || device < 0 || device >= nd) return FFX_ENODEV;
+    ffx_ctx* ctx = new (std::nothrow) ffx_ctx();
+    if (!ctx) return FFX_ENOMEM;
+    ctx->device = device;
+    auto fail = [&](int code) {
+        ffx_ctx_destroy(ctx);
+        return code;
+    };
+    if (hipSetDevice(device) != hipSuccess) return fail(FFX_ENODEV);
+    if (hipStreamCreateWithFlags(&ctx->stream, hipStreamNonBlocking) != hipSuccess) return fail(FFX_EHIP);
+    if (dev_alloc(ctx, &ctx->ctrl, sizeof(Ctrl)) != FFX_OK) return fail(FFX_ENOMEM);
+    if (dev_alloc(ctx, &ctx->total, 64) != FFX_OK) return fail(FFX_ENOMEM);
+    if (dev_alloc(ctx, &ctx->partial, 2 * 1024 * sizeof(double)) != FFX_OK) return fail(FFX_ENOMEM);
+    if (dev_alloc(ctx, &ctx->result, 64) != FFX_OK) return fail(FFX_ENOMEM);
+    if (hipHostMalloc((void**)&ctx->ctrl_host, sizeof(Ctrl), hipHostMallocDefault) != hipSuccess) return fail(FFX_ENOMEM);
+    if (hipHostMalloc((void**)&ctx->result_host, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess)
+        return fail(FFX_ENOMEM);
+    std::memset(ctx->ctrl_host, 0, sizeof(Ctrl));
+    *out = ctx;
+    return FFX_OK;
+}
+
+int ffx_ctx_destroy(ffx_ctx* ctx) {
+    if (!ctx) return FFX_OK;
+    (void)hipSetDevice(ctx->device);
+    if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    free_store(ctx);
+    dev_free(ctx->ctrl);
+    dev_free(ctx->counts);
+    dev_free(ctx->total);
+    dev_free(ctx->partial);
+    dev_free(ctx->result);
+    dev_free(ctx->staging);
+    dev_free(ctx->cliff);
+    if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
+    if (ctx->result_host) (void)hipHostFree(ctx->result_host);
+    if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
+    delete ctx;
+    return FFX_OK;
+}
+
+const char* ffx_last_error(const ffx_ctx* ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void ffx_terms_free(ffx_terms* t) {
+    if (!t) return;
+    free(t->x);
+    free(t->z);
+    free(t->c_re);
+    free(t->c_im);
+    t->x = t->z = nullptr;
+    t->c_re = t->c_im = nullptr;
+    t->n_terms = 0;
+}
+
+int ffx_profile_get(const ffx_ctx* ctx, ffx_profile* out) {
+    if (!ctx || !out) return FFX_EINVAL;
+    *out = ctx->prof;
+    out->launches = ctx->launches;
+    return FFX_OK;
+}
+
+int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                        const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
+                        int64_t n_terms, const ffx_options* opts, ffx_terms* out, const uint64_t* basis_state,
+                        double out_expect[2]) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    if (!opts || n_gates < 0 || n_terms < 0 || (n_gates > 0 && !gates) ||
+        (n_terms > 0 && (!x || !z || !c_re)))
+        return set_err(ctx, FFX_EINVAL, "invalid arguments");
+    if (n_words < 1 || n_words > FFX_MAX_WORDS) return set_err(ctx, FFX_EINVAL, "n_words must be in 1..4");
+    if (n_terms > 0x3FFFFFFF) return set_err(ctx, FFX_ECAPACITY, "too many input terms");
+    for (int64_t i = 0; i < n_gates; ++i) {
+        const ffx_pauli_gate& g = gates[i];
+        if (g.kind < FFX_ROT || g.kind > FFX_CZ) return set_err(ctx, FFX_EINVAL, "unknown gate kind");
+        if (g.kind == FFX_ROT) {
+            for (int w = n_words; w < FFX_MAX_WORDS; ++w)
+                if (g.x[w] || g.z[w]) return set_err(ctx, FFX_EINVAL, "rotation axis exceeds n_words");
+        } else {
+            const int lim = 64 * n_words;
+            const bool two = g.kind >= FFX_CNOT;
+            if (g.q0 < 0 || g.q0 >= lim || (two && (g.q1 < 0 || g.q1 >= lim || g.q1 == g.q0)))
+                return set_err(ctx, FFX_EINVAL, "Clifford qubit out of range");
+        }
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, FFX_ENODEV, "hipSetDevice failed");
+    int K = 1;
+    if (c_im)
+        for (int64_t i = 0; i < n_terms; ++i)
+            if (c_im[i] != 0.0) {
+                K = 2;
+                break;
+            }
+    ctx->prof = ffx_profile{};
+    ctx->launches = 0;
+    const int prec = opts->precision;
+    if (prec < FFX_FP64 || prec > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
+#define FFX_RUNP(W_, CP_, K_)                                                                               \
+    return run_pauli<W_, CP_, K_>(ctx, gates, n_gates, x, z, c_re, K_ == 2 ? c_im : nullptr, n_terms, *opts, \
+                                  out, basis_state, out_expect)
+#define FFX_RUNP_K(W_, CP_)        \
+    if (K == 1) FFX_RUNP(W_, CP_, 1); \
+    FFX_RUNP(W_, CP_, 2)
+#define FFX_RUNP_PREC(W_)                            \
+    if (prec == FFX_FP64) { FFX_RUNP_K(W_, F64); }   \
+    if (prec == FFX_FP32) { FFX_RUNP_K(W_, F32); }   \
+    FFX_RUNP_K(W_, F32C)
+    switch (n_words) {
+        case 1: FFX_RUNP_PREC(1);
+        case 2: FFX_RUNP_PREC(2);
+        case 3: FFX_RUNP_PREC(3);
+        default: FFX_RUNP_PREC(4);
+    }
+#undef FFX_RUNP_PREC
+#undef FFX_RUNP_K
+#undef FFX_RUNP
+}
+
+int ffx_majorana_propagate(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,
+                           const uint64_t* m, const double* c_re, const double* c_im, int64_t n_terms,
+                           int32_t maxdegree, double mincoeff, int32_t precision, ffx_terms* out,
+                           const uint64_t* occ, double out_expect[2]) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    if (n_gates < 0 || n_terms < 0 || (n_gates > 0 && !gates) || (n_terms > 0 && (!m || !c_re)))
+        return set_err(ctx, FFX_EINVAL, "invalid arguments");
+    if (n_words < 1 || n_words > FFX_MAX_WORDS) return set_err(ctx, FFX_EINVAL, "n_words must be in 1..4");
+    if (n_terms > 0x3FFFFFFF) return set_err(ctx, FFX_ECAPACITY, "too many input terms");
+    for (int64_t i = 0; i < n_gates; ++i)
+        for (int w = n_words; w < FFX_MAX_WORDS; ++w)
+            if (gates[i].m[w]) return set_err(ctx, FFX_EINVAL, "rotation axis exceeds n_words");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, FFX_ENODEV, "hipSetDevice failed");
+    // normalized coefficients d = c / i^r are real for every term iff K = 1 suffices
+    int K = 1;
+    for (int64_t i = 0; i < n_terms && K == 1; ++i) {
+        const int r = mono_degree_mod4_r(m + (size_t)i * n_words, n_words);
+        const double dim = r ? -c_re[i] : (c_im ? c_im[i] : 0.0);
+        if (dim != 0.0) K = 2;
+    }
+    ctx->prof = ffx_profile{};
+    ctx->launches = 0;
+    if (precision < FFX_FP64 || precision > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
+#define FFX_RUNM(W_, CP_, K_)                                                                                 \
+    return run_majorana<W_, CP_, K_>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, \
+                                     out_expect)
+#define FFX_RUNM_K(W_, CP_)        \
+    if (K == 1) FFX_RUNM(W_, CP_, 1); \
+    FFX_RUNM(W_, CP_, 2)
+#define FFX_RUNM_PREC(W_)                                 \
+    if (precision == FFX_FP64) { FFX_RUNM_K(W_, F64); }   \
+    if (precision == FFX_FP32) { FFX_RUNM_K(W_, F32); }   \
+    FFX_RUNM_K(W_, F32C)
+    switch (n_words) {
+        case 1: FFX_RUNM_PREC(1);
+        case 2: FFX_RUNM_PREC(2);
+        case 3: FFX_RUNM_PREC(3);
+        default: FFX_RUNM_PREC(4);
+    }
+#undef FFX_RUNM_PREC
+#undef FFX_RUNM_K
+#undef FFX_RUNM
+}
+
+int ffx_expect_last(ffx_ctx* ctx, const uint64_t* state, double out[2]) {
+    if (!ctx || !out) return FFX_EINVAL;
+    ctx->err.clear();
+    if (ctx->last_kind == 0) return set_err(ctx, FFX_EINVAL, "no device-resident result");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, FFX_ENODEV, "hipSetDevice failed");
+    const int W = ctx->last_W, K = ctx->last_K, prec = ctx->last_prec;
+#define FFX_EX(ALG_, W_, CP_, K_) return expect_last_impl<ALG_<W_>, CP_, K_>(ctx, state, out)
+#define FFX_EX_K(ALG_, W_, CP_)   \
+    if (K == 1) FFX_EX(ALG_, W_, CP_, 1); \
+    FFX_EX(ALG_, W_, CP_, 2)
+#define FFX_EX_P(ALG_, W_)                             \
+    if (prec == FFX_FP64) { FFX_EX_K(ALG_, W_, F64); } \
+    if (prec == FFX_FP32) { FFX_EX_K(ALG_, W_, F32); } \
+    FFX_EX_K(ALG_, W_, F32C)
+#define FFX_EX_W(ALG_)              \
+    switch (W) {                    \
+        case 1: FFX_EX_P(ALG_, 1);  \
+        case 2: FFX_EX_P(ALG_, 2);  \
+        case 3: FFX_EX_P(ALG_, 3);  \
+        default: FFX_EX_P(ALG_, 4); \
+    }
+    if (ctx->last_kind == 1) {
+        FFX_EX_W(PauliAlg);
+    }
+    FFX_EX_W(MajoranaAlg);
+#undef FFX_EX_W
+#undef FFX_EX_P
+#undef FFX_EX_K
+#undef FFX_EX
+}
+
+}  // extern "C"

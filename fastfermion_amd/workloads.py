@@ -1,0 +1,96 @@
+"""Synthetic circuits of the benchmark and test workloads, as packed gate tables.
+
+Every random draw comes from SplitMix64 with u = (x >> 11) * 2^-53, so the C++,
+Python and HIP sides can regenerate identical inputs from a seed.
+"""
+from __future__ import annotations
+
+import math
+
+import numpy as np
+
+from ._abi import MAJORANA_GATE_DTYPE, PAULI_GATE_DTYPE, ROT, pauli_word, words_for_bits
+
+
+class SplitMix64:
+    """SplitMix64 (Steele, Lea, Flood 2014)."""
+
+    MASK = (1 << 64) - 1
+
+    def __init__(self, seed: int):
+        self.state = seed & self.MASK
+
+    def next(self) -> int:
+        self.state = (self.state + 0x9E3779B97F4A7C15) & self.MASK
+        z = self.state
+        z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & self.MASK
+        z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & self.MASK
+        return z ^ (z >> 31)
+
+    def uniform(self) -> float:
+        return (self.next() >> 11) * 2.0 ** -53
+
+
+def _pack(gates):
+    out = np.zeros(len(gates), dtype=PAULI_GATE_DTYPE)
+    for i, (ops, qubits, theta) in enumerate(gates):
+        out[i]["kind"] = ROT
+        out[i]["theta"] = theta
+        x, z = pauli_word(ops, qubits)
+        out[i]["x"] = x
+        out[i]["z"] = z
+    return out
+
+
+def tfim_chain(n: int = 20, steps: int = 5, zz: float = 0.2, rx: float = 0.1):
+    """Open-chain transverse-field Ising Trotter circuit: per step RZZ(zz) on the
+    n-1 bonds (i, i+1), then RX(rx) on every qubit."""
+    g = []
+    for _ in range(steps):
+        g += [("ZZ", (i, i + 1), zz) for i in range(n - 1)]
+        g += [("X", (i,), rx) for i in range(n)]
+    return _pack(g)
+
+
+def square_edges(L: int):
+    """Open L x L lattice, site s = L*r + c: horizontal bonds (s, s+1) in
+    row-major order, then vertical bonds (s, s+L)."""
+    h = [(L * r + c, L * r + c + 1) for r in range(L) for c in range(L - 1)]
+    v = [(s, s + L) for s in range(L * (L - 1))]
+    return h + v
+
+
+def kicked_ising(L: int = 10, layers: int = 20, theta_lo: float = 0.2, theta_hi: float = 0.6,
+                 seed: int = 2026, zz: float = -math.pi / 2, thetas=None):
+    """Square-lattice kicked Ising: per layer RZZ(zz) on every edge, then RX(theta_s)
+    on every site; theta_s = lo + (hi-lo)*u_s drawn once per site (seeded)."""
+    n = L * L
+    if thetas is None:
+        rng = SplitMix64(seed)
+        thetas = [theta_lo + (theta_hi - theta_lo) * rng.uniform() for _ in range(n)]
+    edges = square_edges(L)
+    g = []
+    for _ in range(layers):
+        g += [("ZZ", e, zz) for e in edges]
+        g += [("X", (s,), thetas[s]) for s in range(n)]
+    return _pack(g), list(thetas)
+
+
+def single_pauli(ops: str, qubits, nqubits: int, coeff: complex = 1.0):
+    """Observable with one Pauli term, as term-major key arrays."""
+    W = words_for_bits(nqubits)
+    x, z = pauli_word(ops, qubits, W)
+    return x.reshape(1, W), z.reshape(1, W), np.array([coeff], dtype=np.complex128)
+
+
+def pack_majorana(gates):
+    """gates: iterable of (indices, theta, r) -> packed Majorana gate table."""
+    out = np.zeros(len(gates), dtype=MAJORANA_GATE_DTYPE)
+    for i, (idx, theta, r) in enumerate(gates):
+        m = np.zeros(4, dtype=np.uint64)
+        for a in idx:
+            m[a >> 6] |= np.uint64(1) << np.uint64(a & 63)
+        out[i]["m"] = m
+        out[i]["theta"] = theta
+        out[i]["r"] = r
+    return out

@@ -1,0 +1,166 @@
+/*
+ * fastfermion_amd.h -- C ABI of the MI355X propagation library (libffamd.so).
+ *
+ * This is the drop-in boundary for fastfermion's `propagate` hot path.  The
+ * reference exposes that path as the pybind11 function ffcore.propagate with a
+ * Pauli overload (src/python_bind.cpp:392-425 -> src/pauli/propagate.h:297-300)
+ * and a Majorana overload (src/python_bind.cpp:490-527 ->
+ * src/majorana_propagation.h:35-45).  The functions below take exactly the data
+ * those overloads consume -- a gate list and an observable -- as plain arrays,
+ * and return the propagated observable as plain arrays.  No C++ types, no
+ * exceptions and no allocator ownership cross this boundary.
+ *
+ * Conventions
+ *   - Every function returns an int status: FFX_OK (0) or a negative FFX_E*
+ *     code; ffx_last_error(ctx) then holds a message.
+ *   - Input arrays are owned by the caller and only read during the call.
+ *   - Output arrays in ffx_terms are owned by the library; release them with
+ *     ffx_terms_free.
+ *   - A context is bound to one HIP device and owns one HIP stream plus the
+ *     device memory pools; use one context per host thread.
+ *
+ * Key layout (term-major, n_words = W in 1..4, W*64 >= highest index + 1):
+ *   Pauli    x[i*W + w] bit b <=> qubit 64w+b is X or Y  (reference xory)
+ *            z[i*W + w] bit b <=> qubit 64w+b is Y or Z  (reference yorz)
+ *            (src/pauli/algebra.h:30-33)
+ *   Majorana m[i*W + w] bit b <=> Majorana operator m_{64w+b} is in the string
+ *            (src/majorana_algebra.h:30-36)
+ * Keys of one observable must be distinct.
+ */
+#ifndef FASTFERMION_AMD_H
+#define FASTFERMION_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define FFX_MAX_WORDS 4
+
+enum ffx_status {
+    FFX_OK = 0,
+    FFX_EINVAL = -1,    /* invalid argument (shape, index, duplicate keys, ...) */
+    FFX_ENOMEM = -2,    /* device or host allocation failed */
+    FFX_EHIP = -3,      /* HIP runtime error */
+    FFX_ENODEV = -4,    /* no usable HIP device */
+    FFX_ECAPACITY = -5  /* term count exceeds the configured capacity limit */
+};
+
+/* Gate kinds of a Pauli circuit (src/pauli/gates.h:71-269). */
+enum ffx_gate_kind { FFX_ROT = 0, FFX_H = 1, FFX_S = 2, FFX_CNOT = 3, FFX_SWAP = 4, FFX_CZ = 5 };
+
+/* Coefficient arithmetic on the device. */
+enum ffx_precision {
+    FFX_FP64 = 0,   /* IEEE double, same rounding as the reference */
+    FFX_FP32 = 1,   /* float storage and arithmetic */
+    FFX_FP32C = 2   /* float storage with a stored compensation term (float-float) */
+};
+
+/* One gate of a Pauli circuit.  ROT(P, theta) is U = exp(-i theta/2 P)
+ * (src/pauli/gates.h:217-255); Clifford kinds use q0 (and q1 for two-qubit
+ * gates: CNOT control q0 target q1, src/pauli/gates.h:39-51). */
+typedef struct ffx_pauli_gate {
+    int32_t kind;
+    int32_t q0, q1;
+    int32_t reserved;
+    double theta;
+    uint64_t x[FFX_MAX_WORDS];
+    uint64_t z[FFX_MAX_WORDS];
+} ffx_pauli_gate;
+
+/* One Majorana rotation U = exp(-i theta/2 * i^r m_S) (src/majorana_gates.h:17-50);
+ * r = 1 iff deg(S) mod 4 is 2 or 3 (the reference's MROT::_r). */
+typedef struct ffx_majorana_gate {
+    uint64_t m[FFX_MAX_WORDS];
+    double theta;
+    int32_t r;
+    int32_t reserved;
+} ffx_majorana_gate;
+
+/* Truncation and execution options; the first five fields are the reference's
+ * propagate keyword arguments (src/pauli/propagate.h:297-300). */
+typedef struct ffx_options {
+    int32_t maxdegree;      /* partner degree filter (propagate.h:82); < 0: none */
+    int32_t topk;           /* keep the topk largest |c| at each truncation; 0: off */
+    int32_t max_xweight;    /* X/Y-weight cap (truncate.h:41-46); -1: off */
+    int32_t xtrunc_period;  /* the cap applies when (rot_count+1) % period == 0 */
+    int32_t batched;        /* 1: truncate once per commuting run (propagate.h:337-346) */
+    int32_t precision;      /* enum ffx_precision */
+    double mincoeff;        /* erase |c| <= mincoeff after truncation points; 0: off */
+} ffx_options;
+
+/* Propagated observable, term-major, owned by the library. */
+typedef struct ffx_terms {
+    int64_t n_terms;
+    int32_t n_words;
+    int32_t reserved;
+    uint64_t* x;        /* Pauli xory words; Majorana occupation words */
+    uint64_t* z;        /* Pauli yorz words; NULL for Majorana */
+    double* c_re;
+    double* c_im;
+} ffx_terms;
+
+/* Work and timing counters of the last propagate call on a context. */
+typedef struct ffx_profile {
+    int64_t n_gates;         /* rotations applied */
+    int64_t updates;         /* term-gate updates: sum of live terms entering each rotation */
+    int64_t peak_terms;      /* largest live-term count after a rotation */
+    int64_t launches;        /* kernel launches issued */
+    double seconds_total;    /* wall time of the device part (upload to download) */
+    double seconds_upload;
+    double seconds_propagate;
+    double seconds_download;
+} ffx_profile;
+
+typedef struct ffx_ctx ffx_ctx;
+
+/* Context lifetime (no reference counterpart: the reference is in-process CPU code). */
+int ffx_ctx_create(int device, ffx_ctx** out);
+int ffx_ctx_destroy(ffx_ctx* ctx);
+const char* ffx_last_error(const ffx_ctx* ctx);
+
+/* Pauli propagation.  Replaces pauli_gates::propagate (src/pauli/propagate.h:297-493)
+ * behind ffcore.propagate overload #1 (src/python_bind.cpp:392-425).
+ * c_im may be NULL (all imaginary parts zero).  When out_expect is non-NULL the
+ * product-state expectation <b|O|b> with b = basis_state (n_words words; NULL means
+ * |0...0>) is computed on the device; when out is NULL the terms are not
+ * downloaded at all (the result stays on the device until the next call). */
+int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                        const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
+                        int64_t n_terms, const ffx_options* opts, ffx_terms* out,
+                        const uint64_t* basis_state, double out_expect[2]);
+
+/* Majorana propagation.  Replaces majorana_gates::propagate
+ * (src/majorana_propagation.h:18-55) behind ffcore.propagate overload #2
+ * (src/python_bind.cpp:490-527).  maxdegree < 0 means no degree filter; the
+ * threshold applies after every gate when mincoeff > 0.  When out_expect is
+ * non-NULL, <occ|O|occ> for the Fock state with mode occupations occ (NULL: vacuum)
+ * is computed on the device (src/fockstate.h:95-114). */
+int ffx_majorana_propagate(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,
+                           const uint64_t* m, const double* c_re, const double* c_im, int64_t n_terms,
+                           int32_t maxdegree, double mincoeff, int32_t precision, ffx_terms* out,
+                           const uint64_t* occ, double out_expect[2]);
+
+/* Expectation of the device-resident result of the last propagate call on a
+ * product state (Pauli: qubit bits b; Majorana: mode occupations).
+ * Replaces QubitProductState::operator() (src/qubitproductstate.h:82-88),
+ * PauliPolynomial::overlapwithzero (src/pauli/algebra.h:549-557) and
+ * FockState::operator() (src/fockstate.h:95-114). */
+int ffx_expect_last(ffx_ctx* ctx, const uint64_t* state, double out[2]);
+
+void ffx_terms_free(ffx_terms* t);
+
+/* Counters of the last propagate call (the reference's profile_get dictionary,
+ * src/python_bind.cpp:427-446, reports CPU phases; these are device phases). */
+int ffx_profile_get(const ffx_ctx* ctx, ffx_profile* out);
+
+/* Number of visible HIP devices (0 when none); never fails. */
+int ffx_device_count(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

@@ -1,0 +1,895 @@
+// ffo_oracle.cpp -- CPU oracle for truncated Heisenberg-picture propagation.
+//
+// TEST INFRASTRUCTURE ONLY: a plain C++/OpenMP restatement of the reference
+// algorithm, used by tests/ and __graft_entry__.smoke() as the checker and by
+// bench.py as the timed CPU baseline.  The product (fastfermion_amd/) never
+// links or calls it.
+//
+// Every function names the reference lines it restates (paths relative to the
+// reference repository root).  Arithmetic on coefficients is done with
+// std::complex<double> in the same operation order as the reference, so that
+// per-gate values are reproduced exactly; only the container differs (a dense
+// insertion-ordered open-addressing map of our own with the reference map's
+// observable semantics: insertion-order iteration, swap-with-last erase,
+// back-to-front erase_if -- src/hashmap/unordered_dense.h:1052-1086,1983-2000).
+//
+// Parity status of each function (how the oracle itself is pinned) is listed in
+// the header comment of tests/test_oracle.py.
+
+#include "ffo.h"
+
+#include <algorithm>
+#include <chrono>
+#include <cmath>
+#include <complex>
+#include <cstdlib>
+#include <cstring>
+#include <stdexcept>
+#include <string>
+#include <vector>
+
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+namespace {
+
+using cplx = std::complex<double>;
+
+// Work counters of one propagate call (not thread-safe across concurrent calls;
+// the oracle is single-call test infrastructure).
+struct Stats {
+    int64_t updates = 0;
+    int64_t peak = 0;
+    void gate(int64_t n_in, int64_t n_out) {
+        updates += n_in;
+        if (n_out > peak) peak = n_out;
+    }
+};
+Stats g_stats;
+
+inline int popc(uint64_t v) { return __builtin_popcountll(v); }
+
+inline uint64_t mix64(uint64_t h) {
+    // splitmix64 finalizer (Steele, Lea, Flood 2014)
+    h ^= h >> 30;
+    h *= 0xbf58476d1ce4e5b9ULL;
+    h ^= h >> 27;
+    h *= 0x94d049bb133111ebULL;
+    h ^= h >> 31;
+    return h;
+}
+
+// ---------------------------------------------------------------------------
+// Keys
+// ---------------------------------------------------------------------------
+
+// Pauli key: xory/yorz masks (src/pauli/algebra.h:30-33), W 64-bit words each.
+template <int W>
+struct PKey {
+    uint64_t x[W];
+    uint64_t z[W];
+    bool operator==(const PKey& o) const {
+        for (int w = 0; w < W; ++w)
+            if (x[w] != o.x[w] || z[w] != o.z[w]) return false;
+        return true;
+    }
+    uint64_t hash() const {
+        uint64_t h = 0x243f6a8885a308d3ULL;
+        for (int w = 0; w < W; ++w) h = mix64(h ^ x[w]) + z[w];
+        return mix64(h);
+    }
+};
+
+// Majorana key: occupation set (src/majorana_algebra.h:30-36).
+template <int W>
+struct MKey {
+    uint64_t m[W];
+    bool operator==(const MKey& o) const {
+        for (int w = 0; w < W; ++w)
+            if (m[w] != o.m[w]) return false;
+        return true;
+    }
+    uint64_t hash() const {
+        uint64_t h = 0x13198a2e03707344ULL;
+        for (int w = 0; w < W; ++w) h = mix64(h ^ m[w]);
+        return h;
+    }
+};
+
+// ---------------------------------------------------------------------------
+// Dense insertion-ordered hash map (semantics of the reference's ankerl map)
+// ---------------------------------------------------------------------------
+
+template <class Key>
+class DenseMap {
+   public:
+    std::vector<Key> keys;
+    std::vector<cplx> vals;
+
+    size_t size() const { return keys.size(); }
+
+    cplx& operator[](const Key& k) {
+        if ((keys.size() + 1) * 5 > slots_.size() * 4) grow();
+        size_t p = k.hash() & mask_;
+        while (uint32_t s = slots_[p]) {
+            if (keys[s - 1] == k) return vals[s - 1];
+            p = (p + 1) & mask_;
+        }
+        keys.push_back(k);
+        vals.push_back(cplx(0, 0));
+        slots_[p] = static_cast<uint32_t>(keys.size());
+        return vals.back();
+    }
+
+    const cplx* find(const Key& k) const {
+        if (slots_.empty()) return nullptr;
+        size_t p = k.hash() & mask_;
+        while (uint32_t s = slots_[p]) {
+            if (keys[s - 1] == k) return &vals[s - 1];
+            p = (p + 1) & mask_;
+        }
+        return nullptr;
+    }
+
+    void clear() {
+        keys.clear();
+        vals.clear();
+        std::fill(slots_.begin(), slots_.end(), 0u);
+    }
+
+    // std::erase_if on the reference map: back to front, each erased element is
+    // replaced by the current last one (unordered_dense.h:1983-2000, 1052-1086).
+    template <class Pred>
+    size_t erase_if(Pred pred) {
+        size_t old = keys.size();
+        size_t idx = old;
+        while (idx) {
+            --idx;
+            if (pred(keys[idx], vals[idx])) {
+                if (idx != keys.size() - 1) {
+                    keys[idx] = keys.back();
+                    vals[idx] = vals.back();
+                }
+                keys.pop_back();
+                vals.pop_back();
+            }
+        }
+        if (keys.size() != old) rebuild();
+        return old - keys.size();
+    }
+
+    // single erase: the last element moves into the hole (unordered_dense.h:1052-1086)
+    void erase_at(size_t idx) {
+        if (idx != keys.size() - 1) {
+            keys[idx] = keys.back();
+            vals[idx] = vals.back();
+        }
+        keys.pop_back();
+        vals.pop_back();
+        rebuild();
+    }
+
+   private:
+    std::vector<uint32_t> slots_;
+    size_t mask_ = 0;
+
+    void grow() {
+        size_t n = slots_.empty() ? 64 : slots_.size() * 2;
+        slots_.assign(n, 0u);
+        mask_ = n - 1;
+        reindex();
+    }
+    void rebuild() {
+        if (slots_.empty()) return;
+        std::fill(slots_.begin(), slots_.end(), 0u);
+        reindex();
+    }
+    void reindex() {
+        for (size_t i = 0; i < keys.size(); ++i) {
+            size_t p = keys[i].hash() & mask_;
+            while (slots_[p]) p = (p + 1) & mask_;
+            slots_[p] = static_cast<uint32_t>(i + 1);
+        }
+    }
+};
+
+// i^m (src/common.h:57-66)
+inline int m1pow(int m) { return (m % 2 == 0) ? 1 : -1; }
+inline cplx Ipow(int m) {
+    if (m % 2 == 0) return cplx(m1pow(m / 2), 0);
+    return cplx(0, m1pow((m - 1) / 2));
+}
+
+// ---------------------------------------------------------------------------
+// Pauli algebra
+// ---------------------------------------------------------------------------
+
+// PauliString::commutes (src/pauli/algebra.h:209-211)
+template <int W>
+inline bool commutes(const PKey<W>& a, const PKey<W>& b) {
+    int p1 = 0, p2 = 0;
+    for (int w = 0; w < W; ++w) {
+        p1 += popc(a.x[w] & b.z[w]);
+        p2 += popc(a.z[w] & b.x[w]);
+    }
+    return (p1 % 2) == (p2 % 2);
+}
+
+// degree_total (src/pauli/algebra.h:178)
+template <int W>
+inline int degree(const PKey<W>& a) {
+    int d = 0;
+    for (int w = 0; w < W; ++w) d += popc(a.x[w] | a.z[w]);
+    return d;
+}
+
+// degree_x + degree_y (src/pauli/algebra.h:175-176)
+template <int W>
+inline int xweight(const PKey<W>& a) {
+    int d = 0;
+    for (int w = 0; w < W; ++w) d += popc(a.x[w]);
+    return d;
+}
+
+// operator*(PauliString, PauliString) via pauli_string_multiply
+// (src/pauli/algebra.h:227-233, 577-582): product string a*b and phase i^jpow.
+template <int W>
+inline PKey<W> multiply(const PKey<W>& a, const PKey<W>& b, int& jpow) {
+    PKey<W> u;
+    int j = 0;
+    for (int w = 0; w < W; ++w) j += popc(a.x[w] & a.z[w]) + popc(b.x[w] & b.z[w]) + 2 * popc(a.z[w] & b.x[w]);
+    for (int w = 0; w < W; ++w) {
+        u.x[w] = a.x[w] ^ b.x[w];
+        u.z[w] = a.z[w] ^ b.z[w];
+    }
+    for (int w = 0; w < W; ++w) j -= popc(u.x[w] & u.z[w]);
+    jpow = j;
+    return u;
+}
+
+template <int W>
+inline bool getbit(const uint64_t* v, int q) {
+    return (v[q >> 6] >> (q & 63)) & 1ULL;
+}
+template <int W>
+inline void setbit(uint64_t* v, int q, bool b) {
+    uint64_t m = 1ULL << (q & 63);
+    if (b)
+        v[q >> 6] |= m;
+    else
+        v[q >> 6] &= ~m;
+}
+
+// Clifford conjugations H_impl .. CZ_impl (src/pauli/gates.h:15-69), written
+// per qubit bit: each returns the new key and multiplies c by the sign.
+template <int W>
+void clifford_apply(const ffo_pauli_gate& g, PKey<W>& a, cplx& c) {
+    const int i = g.q0, j = g.q1;
+    switch (g.kind) {
+        case FFO_H: {  // X <-> Z, Y -> -Y
+            bool xi = getbit<W>(a.x, i), zi = getbit<W>(a.z, i);
+            setbit<W>(a.x, i, zi);
+            setbit<W>(a.z, i, xi);
+            if (xi && zi) c *= -1;
+            break;
+        }
+        case FFO_S: {  // X -> -Y, Y -> X, Z -> Z
+            bool xi = getbit<W>(a.x, i), zi = getbit<W>(a.z, i);
+            setbit<W>(a.z, i, xi ^ zi);
+            if (xi && !zi) c *= -1;
+            break;
+        }
+        case FFO_CNOT: {  // XI->XX, ZI->ZI, IX->IX, IZ->ZZ
+            bool xi = getbit<W>(a.x, i), zi = getbit<W>(a.z, i);
+            bool xj = getbit<W>(a.x, j), zj = getbit<W>(a.z, j);
+            setbit<W>(a.x, j, xi ^ xj);
+            setbit<W>(a.z, i, zi ^ zj);
+            if (xi && zj && !(zi ^ xj)) c *= -1;
+            break;
+        }
+        case FFO_SWAP: {
+            bool xi = getbit<W>(a.x, i), zi = getbit<W>(a.z, i);
+            bool xj = getbit<W>(a.x, j), zj = getbit<W>(a.z, j);
+            setbit<W>(a.x, i, xj);
+            setbit<W>(a.x, j, xi);
+            setbit<W>(a.z, i, zj);
+            setbit<W>(a.z, j, zi);
+            break;
+        }
+        case FFO_CZ: {  // XI->XZ, IX->ZX
+            bool xi = getbit<W>(a.x, i), zi = getbit<W>(a.z, i);
+            bool xj = getbit<W>(a.x, j), zj = getbit<W>(a.z, j);
+            if (xi && xj && (zi ^ zj)) c *= -1;
+            setbit<W>(a.z, i, zi ^ xj);
+            setbit<W>(a.z, j, zj ^ xi);
+            break;
+        }
+        default:
+            throw std::runtime_error("not a Clifford gate");
+    }
+}
+
+template <int W>
+using PMap = DenseMap<PKey<W>>;
+
+template <int W>
+struct Rot {
+    PKey<W> p;
+    double theta;
+};
+
+template <int W>
+Rot<W> to_rot(const ffo_pauli_gate& g) {
+    Rot<W> r;
+    for (int w = 0; w < W; ++w) {
+        r.p.x[w] = g.x[w];
+        r.p.z[w] = g.z[w];
+    }
+    r.theta = g.theta;
+    return r;
+}
+
+// _apply_clifford_circuit (src/pauli/gates.h:273-301): gates [begin, end) are
+// applied to every term from the last one to the first; the map is rebuilt.
+template <int W>
+void apply_clifford_block(PMap<W>& obs, const ffo_pauli_gate* gates, int64_t begin, int64_t end) {
+    PMap<W> result;
+    for (size_t t = 0; t < obs.size(); ++t) {
+        PKey<W> y = obs.keys[t];
+        cplx mult = 1;
+        for (int64_t i = end - 1; i >= begin; --i) clifford_apply<W>(gates[i], y, mult);
+        result[y] += mult * obs.vals[t];
+    }
+    obs = std::move(result);
+}
+
+// conjugate (src/pauli/propagate.h:71-89)
+template <int W>
+void conjugate(PMap<W>& obs, const Rot<W>& gate, int maxdegree) {
+    std::vector<std::pair<PKey<W>, cplx>> new_terms;
+    new_terms.reserve(obs.size());
+    const double cos_t = cos(gate.theta);
+    const cplx isin_t = cplx(0, sin(gate.theta));
+    for (size_t t = 0; t < obs.size(); ++t) {
+        const PKey<W>& x = obs.keys[t];
+        if (!commutes<W>(x, gate.p)) {
+            int jpow;
+            PKey<W> partner = multiply<W>(gate.p, x, jpow);
+            if (degree<W>(partner) <= maxdegree)
+                new_terms.emplace_back(partner, obs.vals[t] * isin_t * Ipow(jpow));
+            obs.vals[t] *= cos_t;
+        }
+    }
+    for (const auto& nt : new_terms) obs[nt.first] += nt.second;
+}
+
+// truncate_threshold / truncate_top_k / truncate_x_weight / truncate_all
+// (src/pauli/truncate.h:16-57)
+template <int W>
+void truncate_all(PMap<W>& obs, double mincoeff, int topk, int max_xweight, int xtrunc_period,
+                  int rot_count) {
+    if (mincoeff > 0)
+        obs.erase_if([mincoeff](const PKey<W>&, const cplx& c) { return std::abs(c) <= mincoeff; });
+    if (topk > 0 && (int)obs.size() > topk) {
+        std::vector<double> mags;
+        mags.reserve(obs.size());
+        for (const auto& c : obs.vals) mags.push_back(std::abs(c));
+        std::nth_element(mags.begin(), mags.begin() + topk, mags.end(), std::greater<double>());
+        const double cutoff = mags[topk];
+        obs.erase_if([cutoff](const PKey<W>&, const cplx& c) { return std::abs(c) < cutoff; });
+        // trim exact ties at the boundary: repeatedly erase the first entry (in
+        // iteration order) that is not above the cutoff (truncate.h:31-37)
+        while ((int)obs.size() > topk) {
+            size_t it = 0;
+            while (it < obs.size() && std::abs(obs.vals[it]) > cutoff) ++it;
+            if (it == obs.size()) break;
+            obs.erase_at(it);
+        }
+    }
+    if (max_xweight >= 0 && xtrunc_period > 0 && (rot_count + 1) % xtrunc_period == 0)
+        obs.erase_if([max_xweight](const PKey<W>& k, const cplx&) { return xweight<W>(k) > max_xweight; });
+}
+
+// batch_commuting_gates (src/pauli/propagate.h:272-291)
+template <int W>
+std::vector<std::vector<Rot<W>>> batch_commuting_gates(const std::vector<Rot<W>>& gates) {
+    std::vector<std::vector<Rot<W>>> batches;
+    std::vector<Rot<W>> current;
+    for (const auto& g : gates) {
+        bool ok = true;
+        for (const auto& h : current)
+            if (!commutes<W>(g.p, h.p)) {
+                ok = false;
+                break;
+            }
+        if (ok)
+            current.push_back(g);
+        else {
+            if (!current.empty()) batches.push_back(std::move(current));
+            current = {g};
+        }
+    }
+    if (!current.empty()) batches.push_back(std::move(current));
+    return batches;
+}
+
+// ---------------------------------------------------------------------------
+// Sharded strategy (src/pauli/propagate.h:157-264)
+// ---------------------------------------------------------------------------
+
+template <int W>
+using Sharded = std::vector<PMap<W>>;
+
+template <int W>
+inline int shard_of(const PKey<W>& k, int n) {
+    return static_cast<int>(mix64(k.hash() ^ 0x5bd1e995ULL) % static_cast<uint64_t>(n));
+}
+
+template <int W>
+Sharded<W> to_sharded(const PMap<W>& poly, int n) {
+    Sharded<W> s(n);
+    for (size_t t = 0; t < poly.size(); ++t) s[shard_of<W>(poly.keys[t], n)][poly.keys[t]] += poly.vals[t];
+    return s;
+}
+
+template <int W>
+PMap<W> from_sharded(const Sharded<W>& s) {
+    PMap<W> out;
+    for (const auto& sh : s)
+        for (size_t t = 0; t < sh.size(); ++t) out[sh.keys[t]] += sh.vals[t];
+    return out;
+}
+
+template <int W>
+void conjugate_sharded(Sharded<W>& shards, const Rot<W>& gate, int maxdegree, int n_threads) {
+    const double cos_t = cos(gate.theta);
+    const cplx isin_t = cplx(0, sin(gate.theta));
+    using SendBuf = std::vector<std::pair<PKey<W>, cplx>>;
+    std::vector<std::vector<SendBuf>> outgoing(n_threads, std::vector<SendBuf>(n_threads));
+#pragma omp parallel num_threads(n_threads)
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+        auto& shard = shards[tid];
+        SendBuf local_new;
+        for (size_t t = 0; t < shard.size(); ++t) {
+            const PKey<W>& x = shard.keys[t];
+            if (!commutes<W>(x, gate.p)) {
+                int jpow;
+                PKey<W> pk = multiply<W>(gate.p, x, jpow);
+                if (degree<W>(pk) <= maxdegree) {
+                    cplx pc = shard.vals[t] * isin_t * Ipow(jpow);
+                    int dest = shard_of<W>(pk, n_threads);
+                    if (dest == tid)
+                        local_new.emplace_back(pk, pc);
+                    else
+                        outgoing[tid][dest].emplace_back(pk, pc);
+                }
+                shard.vals[t] *= cos_t;
+            }
+        }
+        for (const auto& kv : local_new) shard[kv.first] += kv.second;
+    }
+#pragma omp parallel num_threads(n_threads)
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+        auto& shard = shards[tid];
+        for (int src = 0; src < n_threads; ++src)
+            for (const auto& kv : outgoing[src][tid]) shard[kv.first] += kv.second;
+    }
+}
+
+template <int W>
+void truncate_sharded(Sharded<W>& shards, int n_threads, double mincoeff, int topk, int max_xweight,
+                      int xtrunc_period, int rot_count) {
+    if (topk > 0) {
+        PMap<W> merged = from_sharded<W>(shards);
+        truncate_all<W>(merged, mincoeff, topk, max_xweight, xtrunc_period, rot_count);
+        shards = to_sharded<W>(merged, n_threads);
+        return;
+    }
+#pragma omp parallel num_threads(n_threads)
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+        truncate_all<W>(shards[tid], mincoeff, 0, max_xweight, xtrunc_period, rot_count);
+    }
+}
+
+// conjugate_omp (src/pauli/propagate.h:97-146): parallel emission, serial rebuild.
+template <int W>
+void conjugate_omp(PMap<W>& obs, const Rot<W>& gate, int maxdegree, int n_threads) {
+    std::vector<std::pair<PKey<W>, cplx>> snap;
+    snap.reserve(obs.size());
+    for (size_t t = 0; t < obs.size(); ++t) snap.emplace_back(obs.keys[t], obs.vals[t]);
+    const double cos_t = cos(gate.theta);
+    const cplx isin_t = cplx(0, sin(gate.theta));
+    std::vector<std::vector<std::pair<PKey<W>, cplx>>> kept(n_threads), partners(n_threads);
+#pragma omp parallel num_threads(n_threads)
+    {
+#ifdef _OPENMP
+        int tid = omp_get_thread_num();
+#else
+        int tid = 0;
+#endif
+#pragma omp for schedule(static)
+        for (long j = 0; j < (long)snap.size(); ++j) {
+            const auto& q = snap[j];
+            if (commutes<W>(q.first, gate.p))
+                kept[tid].emplace_back(q.first, q.second);
+            else {
+                kept[tid].emplace_back(q.first, q.second * cos_t);
+                int jpow;
+                PKey<W> pk = multiply<W>(gate.p, q.first, jpow);
+                if (degree<W>(pk) <= maxdegree) partners[tid].emplace_back(pk, q.second * isin_t * Ipow(jpow));
+            }
+        }
+    }
+    obs.clear();
+    for (int t = 0; t < n_threads; ++t) {
+        for (const auto& kv : kept[t]) obs[kv.first] += kv.second;
+        for (const auto& kv : partners[t]) obs[kv.first] += kv.second;
+    }
+}
+
+// propagate (src/pauli/propagate.h:297-493): reverse walk, Clifford blocks,
+// commuting batches, strategy dispatch.
+template <int W>
+PMap<W> pauli_propagate(const ffo_pauli_gate* circuit, int64_t n, PMap<W> obs, const ffo_options& o,
+                        int strategy, int n_threads) {
+#ifndef _OPENMP
+    n_threads = 1;
+#endif
+    if (n_threads <= 1) strategy = FFO_SERIAL;
+    const int maxdegree = o.maxdegree < 0 ? 1 << 30 : o.maxdegree;
+    int64_t clifford_begin = 0;
+    bool pending = false;
+    int rot_count = 0;
+    const bool sharded = strategy == FFO_SHARDED;
+    Sharded<W> shards;
+    if (sharded) shards = to_sharded<W>(obs, n_threads);
+
+    auto live = [&]() -> int64_t {
+        if (!sharded) return static_cast<int64_t>(obs.size());
+        int64_t n = 0;
+        for (const auto& sh : shards) n += static_cast<int64_t>(sh.size());
+        return n;
+    };
+    auto do_conj = [&](const Rot<W>& g) {
+        const int64_t n_in = live();
+        if (strategy == FFO_SERIAL)
+            conjugate<W>(obs, g, maxdegree);
+        else if (strategy == FFO_OMP)
+            conjugate_omp<W>(obs, g, maxdegree, n_threads);
+        else
+            conjugate_sharded<W>(shards, g, maxdegree, n_threads);
+        g_stats.gate(n_in, live());
+    };
+    auto do_trunc = [&](int rc) {
+        if (sharded)
+            truncate_sharded<W>(shards, n_threads, o.mincoeff, o.topk, o.max_xweight, o.xtrunc_period, rc);
+        else
+            truncate_all<W>(obs, o.mincoeff, o.topk, o.max_xweight, o.xtrunc_period, rc);
+    };
+    auto do_clifford = [&](int64_t b, int64_t e) {
+        if (sharded) obs = from_sharded<W>(shards);
+        apply_clifford_block<W>(obs, circuit, b, e);
+        if (sharded) shards = to_sharded<W>(obs, n_threads);
+    };
+
+    std::vector<Rot<W>> rot_buffer;
+    auto flush = [&]() {
+        if (rot_buffer.empty()) return;
+        for (const auto& batch : batch_commuting_gates<W>(rot_buffer)) {
+            for (const auto& g : batch) do_conj(g);
+            rot_count += static_cast<int>(batch.size());
+            do_trunc(rot_count - 1);
+        }
+        rot_buffer.clear();
+    };
+    for (int64_t i = n - 1; i >= 0; --i) {
+        if (circuit[i].kind != FFO_ROT) {
+            if (o.batched) flush();
+            if (!pending) {
+                clifford_begin = i;
+                pending = true;
+            }
+        } else {
+            if (pending) {
+                do_clifford(i + 1, clifford_begin + 1);
+                pending = false;
+            }
+            if (o.batched)
+                rot_buffer.push_back(to_rot<W>(circuit[i]));
+            else {
+                do_conj(to_rot<W>(circuit[i]));
+                do_trunc(rot_count);
+                rot_count++;
+            }
+        }
+    }
+    if (o.batched) flush();
+    if (pending) {
+        if (sharded) obs = from_sharded<W>(shards);
+        apply_clifford_block<W>(obs, circuit, 0, clifford_begin + 1);
+        return obs;
+    }
+    if (sharded) return from_sharded<W>(shards);
+    return obs;
+}
+
+// ---------------------------------------------------------------------------
+// Majorana algebra and propagation
+// ---------------------------------------------------------------------------
+
+// MajoranaString::commutes (src/majorana_algebra.h:79-84)
+template <int W>
+inline bool mcommutes(const MKey<W>& a, const MKey<W>& b) {
+    int na = 0, nb = 0, nab = 0;
+    for (int w = 0; w < W; ++w) {
+        na += popc(a.m[w]);
+        nb += popc(b.m[w]);
+        nab += popc(a.m[w] & b.m[w]);
+    }
+    return ((na * nb - nab) % 2) == 0;
+}
+
+template <int W>
+inline int mdegree(const MKey<W>& a) {
+    int d = 0;
+    for (int w = 0; w < W; ++w) d += popc(a.m[w]);
+    return d;
+}
+
+// merge_parity(x, y) (src/bits_utils.h:14-38): parity of #{(i in x, j in y): i < j}.
+// Restated literally: for every set bit j of y, count the bits of x below j.
+template <int W>
+inline int merge_parity(const MKey<W>& x, const MKey<W>& y) {
+    int tot = 0;
+    for (int w = 0; w < W; ++w) {
+        uint64_t v = y.m[w];
+        while (v) {
+            int b = __builtin_ctzll(v);
+            v &= v - 1;
+            int j = 64 * w + b;
+            for (int u = 0; u < W; ++u) {
+                uint64_t below;
+                if (64 * u + 63 < j)
+                    below = ~0ULL;
+                else if (64 * u >= j)
+                    below = 0;
+                else
+                    below = (1ULL << (j - 64 * u)) - 1;
+                tot += popc(x.m[u] & below);
+            }
+        }
+    }
+    return tot % 2;
+}
+
+template <int W>
+using MMap = DenseMap<MKey<W>>;
+
+// MROT::apply_inplace (src/majorana_gates.h:67-103) followed by the threshold of
+// _propagate (src/majorana_propagation.h:18-33).
+template <int W>
+MMap<W> majorana_propagate(const ffo_majorana_gate* circuit, int64_t n, MMap<W> ret, int maxdegree,
+                           double mincoeff) {
+    for (int64_t i = n - 1; i >= 0; --i) {
+        const ffo_majorana_gate& g = circuit[i];
+        const int64_t n_in = static_cast<int64_t>(ret.size());
+        MKey<W> ms;
+        for (int w = 0; w < W; ++w) ms.m[w] = g.m[w];
+        std::vector<std::pair<MKey<W>, cplx>> o_new;
+        o_new.reserve(ret.size());
+        const double costheta = cos(g.theta);
+        const cplx iirsintheta = g.r ? cplx(-sin(g.theta), 0) : cplx(0, sin(g.theta));
+        for (size_t t = 0; t < ret.size(); ++t) {
+            const MKey<W>& x = ret.keys[t];
+            if (!mcommutes<W>(x, ms)) {
+                // operator*(ms, x) (src/majorana_algebra.h:441-444)
+                MKey<W> prod;
+                for (int w = 0; w < W; ++w) prod.m[w] = ms.m[w] ^ x.m[w];
+                cplx mxcoef(m1pow(merge_parity<W>(x, ms)), 0);
+                if (maxdegree < 0 || mdegree<W>(prod) <= maxdegree)
+                    o_new.emplace_back(prod, ret.vals[t] * iirsintheta * mxcoef);
+                ret.vals[t] *= costheta;
+            }
+        }
+        for (const auto& kv : o_new) ret[kv.first] += kv.second;
+        g_stats.gate(n_in, static_cast<int64_t>(ret.size()));
+        if (mincoeff > 0)
+            ret.erase_if([mincoeff](const MKey<W>&, const cplx& c) { return std::abs(c) <= mincoeff; });
+    }
+    return ret;
+}
+
+// ---------------------------------------------------------------------------
+// C interface plumbing
+// ---------------------------------------------------------------------------
+
+template <int W>
+int run_pauli(const ffo_pauli_gate* gates, int64_t n_gates, const uint64_t* x, const uint64_t* z,
+              const double* cre, const double* cim, int64_t n_terms, const ffo_options* o, int strategy,
+              int n_threads, ffo_terms* out) {
+    PMap<W> obs;
+    for (int64_t t = 0; t < n_terms; ++t) {
+        PKey<W> k;
+        for (int w = 0; w < W; ++w) {
+            k.x[w] = x[t * W + w];
+            k.z[w] = z[t * W + w];
+        }
+        obs[k] += cplx(cre ? cre[t] : 0.0, cim ? cim[t] : 0.0);
+    }
+    g_stats = Stats();
+    auto t0 = std::chrono::steady_clock::now();
+    PMap<W> res = pauli_propagate<W>(gates, n_gates, std::move(obs), *o, strategy, n_threads);
+    auto t1 = std::chrono::steady_clock::now();
+    const size_t n = res.size();
+    out->n_terms = static_cast<int64_t>(n);
+    out->n_words = W;
+    out->x = static_cast<uint64_t*>(malloc(sizeof(uint64_t) * W * (n ? n : 1)));
+    out->z = static_cast<uint64_t*>(malloc(sizeof(uint64_t) * W * (n ? n : 1)));
+    out->c_re = static_cast<double*>(malloc(sizeof(double) * (n ? n : 1)));
+    out->c_im = static_cast<double*>(malloc(sizeof(double) * (n ? n : 1)));
+    if (!out->x || !out->z || !out->c_re || !out->c_im) return -2;
+    for (size_t t = 0; t < n; ++t) {
+        for (int w = 0; w < W; ++w) {
+            out->x[t * W + w] = res.keys[t].x[w];
+            out->z[t * W + w] = res.keys[t].z[w];
+        }
+        out->c_re[t] = res.vals[t].real();
+        out->c_im[t] = res.vals[t].imag();
+    }
+    out->seconds = std::chrono::duration<double>(t1 - t0).count();
+    out->updates = g_stats.updates;
+    out->peak_terms = g_stats.peak;
+    return 0;
+}
+
+template <int W>
+int run_majorana(const ffo_majorana_gate* gates, int64_t n_gates, const uint64_t* m, const double* cre,
+                 const double* cim, int64_t n_terms, int maxdegree, double mincoeff, ffo_terms* out) {
+    MMap<W> obs;
+    for (int64_t t = 0; t < n_terms; ++t) {
+        MKey<W> k;
+        for (int w = 0; w < W; ++w) k.m[w] = m[t * W + w];
+        obs[k] += cplx(cre ? cre[t] : 0.0, cim ? cim[t] : 0.0);
+    }
+    g_stats = Stats();
+    auto t0 = std::chrono::steady_clock::now();
+    MMap<W> res = majorana_propagate<W>(gates, n_gates, std::move(obs), maxdegree, mincoeff);
+    auto t1 = std::chrono::steady_clock::now();
+    const size_t n = res.size();
+    out->n_terms = static_cast<int64_t>(n);
+    out->n_words = W;
+    out->x = static_cast<uint64_t*>(malloc(sizeof(uint64_t) * W * (n ? n : 1)));
+    out->z = nullptr;
+    out->c_re = static_cast<double*>(malloc(sizeof(double) * (n ? n : 1)));
+    out->c_im = static_cast<double*>(malloc(sizeof(double) * (n ? n : 1)));
+    if (!out->x || !out->c_re || !out->c_im) return -2;
+    for (size_t t = 0; t < n; ++t) {
+        for (int w = 0; w < W; ++w) out->x[t * W + w] = res.keys[t].m[w];
+        out->c_re[t] = res.vals[t].real();
+        out->c_im[t] = res.vals[t].imag();
+    }
+    out->seconds = std::chrono::duration<double>(t1 - t0).count();
+    out->updates = g_stats.updates;
+    out->peak_terms = g_stats.peak;
+    return 0;
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffo_pauli_propagate(const ffo_pauli_gate* gates, int64_t n_gates, int32_t n_words, const uint64_t* x,
+                        const uint64_t* z, const double* c_re, const double* c_im, int64_t n_terms,
+                        const ffo_options* opts, int32_t strategy, int32_t n_threads, ffo_terms* out) {
+    if (!out || !opts || n_gates < 0 || n_terms < 0) return -1;
+    std::memset(out, 0, sizeof(*out));
+    try {
+        switch (n_words) {
+            case 1: return run_pauli<1>(gates, n_gates, x, z, c_re, c_im, n_terms, opts, strategy, n_threads, out);
+            case 2: return run_pauli<2>(gates, n_gates, x, z, c_re, c_im, n_terms, opts, strategy, n_threads, out);
+            case 3: return run_pauli<3>(gates, n_gates, x, z, c_re, c_im, n_terms, opts, strategy, n_threads, out);
+            case 4: return run_pauli<4>(gates, n_gates, x, z, c_re, c_im, n_terms, opts, strategy, n_threads, out);
+            default: return -1;
+        }
+    } catch (...) {
+        return -3;
+    }
+}
+
+int ffo_majorana_propagate(const ffo_majorana_gate* gates, int64_t n_gates, int32_t n_words, const uint64_t* m,
+                           const double* c_re, const double* c_im, int64_t n_terms, int32_t maxdegree,
+                           double mincoeff, ffo_terms* out) {
+    if (!out || n_gates < 0 || n_terms < 0) return -1;
+    std::memset(out, 0, sizeof(*out));
+    try {
+        switch (n_words) {
+            case 1: return run_majorana<1>(gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out);
+            case 2: return run_majorana<2>(gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out);
+            case 3: return run_majorana<3>(gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out);
+            case 4: return run_majorana<4>(gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out);
+            default: return -1;
+        }
+    } catch (...) {
+        return -3;
+    }
+}
+
+// QubitProductState::operator() (src/qubitproductstate.h:73-88)
+void ffo_pauli_expect(const ffo_terms* t, const uint64_t* b, double out[2]) {
+    cplx ret = 0;
+    const int W = t->n_words;
+    for (int64_t i = 0; i < t->n_terms; ++i) {
+        bool diag = true;
+        int par = 0;
+        for (int w = 0; w < W; ++w) {
+            if (t->x[i * W + w]) diag = false;
+            par += popc(t->z[i * W + w] & b[w]);
+        }
+        if (diag) ret += cplx(t->c_re[i], t->c_im[i]) * cplx(m1pow(par), 0);
+    }
+    out[0] = ret.real();
+    out[1] = ret.imag();
+}
+
+// FockState::operator()(MajoranaString) (src/fockstate.h:95-114): nonzero only
+// for fully paired strings {2j, 2j+1 : j in J}; value i^(|J| + 2|b & J|).
+void ffo_majorana_expect(const ffo_terms* t, const uint64_t* occ, double out[2]) {
+    cplx ret = 0;
+    const int W = t->n_words;
+    for (int64_t i = 0; i < t->n_terms; ++i) {
+        bool paired = true;
+        int nj = 0, nbj = 0;
+        for (int w = 0; w < W && paired; ++w) {
+            uint64_t a = t->x[i * W + w];
+            uint64_t ev = a & 0x5555555555555555ULL, od = (a & 0xAAAAAAAAAAAAAAAAULL) >> 1;
+            if (ev != od) paired = false;
+            for (int b = 0; b < 64; b += 2) {
+                if ((ev >> b) & 1ULL) {
+                    int mode = (64 * w + b) / 2;
+                    nj++;
+                    if ((occ[mode >> 6] >> (mode & 63)) & 1ULL) nbj++;
+                }
+            }
+        }
+        if (paired) ret += cplx(t->c_re[i], t->c_im[i]) * Ipow(nj + 2 * nbj);
+    }
+    out[0] = ret.real();
+    out[1] = ret.imag();
+}
+
+void ffo_free(ffo_terms* t) {
+    if (!t) return;
+    free(t->x);
+    free(t->z);
+    free(t->c_re);
+    free(t->c_im);
+    t->x = t->z = nullptr;
+    t->c_re = t->c_im = nullptr;
+    t->n_terms = 0;
+}
+
+int32_t ffo_max_threads(void) {
+#ifdef _OPENMP
+    return omp_get_max_threads();
+#else
+    return 0;
+#endif
+}
+
+}  // extern "C"
