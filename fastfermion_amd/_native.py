@@ -23,7 +23,7 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP error",
 
 EXPORTED = ("ffx_ctx_create", "ffx_ctx_destroy", "ffx_last_error", "ffx_pauli_propagate",
             "ffx_majorana_propagate", "ffx_expect_last", "ffx_terms_free", "ffx_profile_get",
-            "ffx_device_count")
+            "ffx_device_count", "ffx_set_kernel_timing")
 
 
 class Options(ctypes.Structure):
@@ -43,7 +43,9 @@ class Profile(ctypes.Structure):
     _fields_ = [("n_gates", ctypes.c_int64), ("updates", ctypes.c_int64), ("peak_terms", ctypes.c_int64),
                 ("launches", ctypes.c_int64), ("seconds_total", ctypes.c_double),
                 ("seconds_upload", ctypes.c_double), ("seconds_propagate", ctypes.c_double),
-                ("seconds_download", ctypes.c_double)]
+                ("seconds_download", ctypes.c_double), ("outs", ctypes.c_int64),
+                ("compactions", ctypes.c_int64), ("run_launches", ctypes.c_int64),
+                ("seconds_kernel", ctypes.c_double)]
 
 
 _lib = None
@@ -71,6 +73,7 @@ def load():
             L.ffx_terms_free.argtypes = [ctypes.POINTER(Terms)]
             L.ffx_profile_get.argtypes = [vp, ctypes.POINTER(Profile)]
             L.ffx_device_count.argtypes = []
+            L.ffx_set_kernel_timing.argtypes = [vp, ctypes.c_int]
             _lib = L
     return _lib
 
@@ -200,6 +203,10 @@ class Context:
         ex = np.zeros(2)
         self._check(self._L.ffx_expect_last(self._h, _p(st), _p(ex)))
         return complex(ex[0], ex[1])
+
+    def set_kernel_timing(self, on: bool) -> None:
+        """Bracket every propagation kernel launch with HIP events (see profile())."""
+        self._check(self._L.ffx_set_kernel_timing(self._h, int(bool(on))))
 
     def profile(self) -> dict:
         p = Profile()

@@ -1,0 +1,289 @@
+// ffx_async.h -- kernels of the asynchronous engine: one launch per step, no
+// host synchronisation between steps.
+//
+// Kernel boundaries are the global barrier (on MI355X a launch gap costs about
+// 2.5 us, against 4..70 us for an in-kernel grid barrier over 64..1024
+// blocks).  The term count and erased count flow from step to step through
+// per-epoch arrays in device memory: a step kernel for epoch e reads
+//   n = n[e-1] + app[e-1],   dead = dead[e-1] + dd[e-1],
+// records them in n[e], dead[e] (block 0, thread 0; read by later kernels
+// only), and accumulates its own appends and erased-count change in app[e] and
+// dd[e].  No counter is reset inside the run: the host zeroes app[] and dd[]
+// once.  Appends are block-aggregated: one atomicAdd per block and loop
+// iteration (a single contended address serialises at ~11 ns per atomic).
+//
+// A step whose appends would exceed the capacity sets flags[0] and records its
+// epoch in flags[1]; every later kernel returns at entry, so the host can roll
+// back that step's index entries, grow the store and resume from that step.
+#pragma once
+
+#include "ffx_kernels.h"
+
+namespace ffx {
+
+constexpr int kBlockRot = 1024;
+
+struct Epochs {
+    uint32_t* n;     // slots in use at the start of the epoch's kernel
+    uint32_t* dead;  // erased slots at the start
+    uint32_t* app;   // appends of the epoch
+    int32_t* dd;     // erased-count change of the epoch
+    uint32_t* flags; // [0] overflow raised, [1] epoch that raised it
+};
+
+__device__ __forceinline__ bool run_stopped(const Epochs& ep) { return atomic_load_u32(&ep.flags[0]) != 0; }
+
+__device__ __forceinline__ void epoch_in(const Epochs& ep, uint32_t e, uint32_t& n, uint32_t& dead) {
+    n = ep.n[e - 1] + ep.app[e - 1];
+    dead = (uint32_t)((int32_t)ep.dead[e - 1] + ep.dd[e - 1]);
+}
+
+// One rotation.  Each block owns a contiguous chunk of slots.
+template <class Alg, class CP, int K>
+__global__ void __launch_bounds__(kBlockRot) k_rot_async(Store<CP, K> st, const typename CP::store_t* cin,
+                                                         typename CP::store_t* cout, typename Alg::Gate g,
+                                                         RotParams<CP> rp, Epochs ep, uint32_t e) {
+    constexpr int NP = Alg::NP;
+    using val_t = typename CP::val_t;
+    if (run_stopped(ep)) return;
+    uint32_t n, dead;
+    epoch_in(ep, e, n, dead);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ep.n[e] = n;
+        ep.dead[e] = dead;
+    }
+    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
+    const uint32_t chunk = (per + blockDim.x - 1) / blockDim.x * blockDim.x;
+    const uint32_t begin = blockIdx.x * chunk;
+    const uint32_t end = min(n, begin + chunk);
+    __shared__ uint32_t sbase;
+    int dd = 0;
+    for (uint32_t b0 = begin; b0 < end; b0 += blockDim.x) {
+        const uint32_t i = b0 + threadIdx.x;
+        uint64_t r[NP];
+        val_t vapp[K];
+        bool need_app = false;
+        if (i < end) rot_slot<Alg, CP, K>(st, st.key, cin, cout, g, rp, n, i, need_app, r, vapp, dd);
+        uint32_t tot;
+        const uint32_t rank = block_exclusive_scan(need_app ? 1u : 0u, &tot);
+        if (tot) {
+            if (threadIdx.x == 0) sbase = atomicAdd(&ep.app[e], tot);
+            __syncthreads();
+            if (need_app) {
+                const uint64_t slot = (uint64_t)n + sbase + rank;
+                if (slot < st.cap) {
+                    put_slot<NP, CP, K>(st, st.key, cout, slot, r, vapp);
+                } else {
+                    atomicOr(&ep.flags[0], 1u);
+                    ep.flags[1] = e;
+                }
+            }
+        }
+    }
+    dd = block_sum_int(dd);
+    if (threadIdx.x == 0 && dd) atomicAdd(&ep.dd[e], dd);
+}
+
+// ---- compaction sequence (device-decided: runs when erased > n/4) ----
+__device__ __forceinline__ bool compact_wanted(uint32_t n, uint32_t dead) {
+    return dead > 0 && (uint64_t)dead * 4 > n;
+}
+
+template <class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_cmp_count(Store<CP, K> st, Epochs ep, uint32_t e_prev, uint32_t* counts) {
+    if (run_stopped(ep)) return;
+    const uint32_t n = ep.n[e_prev] + ep.app[e_prev];
+    const uint32_t dead = (uint32_t)((int32_t)ep.dead[e_prev] + ep.dd[e_prev]);
+    if (!compact_wanted(n, dead)) return;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        int c = 0;
+#pragma unroll
+        for (int it = 0; it < kItems; ++it) {
+            const uint32_t i = t * kTile + it * kBlock + threadIdx.x;
+            if (i < n && !CP::dead(st.ld(st.coef, 0, i))) ++c;
+        }
+        c = block_sum_int(c);
+        if (threadIdx.x == 0) counts[t] = (uint32_t)c;
+    }
+}
+
+__global__ void __launch_bounds__(kBlock) k_cmp_scan(Epochs ep, uint32_t e_prev, uint32_t* counts) {
+    if (run_stopped(ep)) return;
+    const uint32_t n = ep.n[e_prev] + ep.app[e_prev];
+    const uint32_t dead = (uint32_t)((int32_t)ep.dead[e_prev] + ep.dd[e_prev]);
+    if (!compact_wanted(n, dead)) return;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < tiles; b0 += kBlock) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < tiles ? counts[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(v, &tot);
+        if (i < tiles) counts[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) counts[tiles] = carry;
+}
+
+// Scatter live slots into key2/coef2 and record the epoch's (n, dead).
+template <int NP, class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_cmp_scatter(Store<CP, K> st, uint64_t* key2,
+                                                        typename CP::store_t* coef2, Epochs ep, uint32_t e_prev,
+                                                        uint32_t e, const uint32_t* counts) {
+    if (run_stopped(ep)) return;
+    const uint32_t n = ep.n[e_prev] + ep.app[e_prev];
+    const uint32_t dead = (uint32_t)((int32_t)ep.dead[e_prev] + ep.dd[e_prev]);
+    const bool want = compact_wanted(n, dead);
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ep.n[e] = want ? counts[tiles] : n;
+        ep.dead[e] = want ? 0u : dead;
+    }
+    if (!want) return;
+    const size_t cap = st.cap;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        uint32_t run = counts[t];
+        for (int it = 0; it < kItems; ++it) {
+            const uint32_t i = t * kTile + it * kBlock + threadIdx.x;
+            bool live = false;
+            if (i < n) live = !CP::dead(st.ld(st.coef, 0, i));
+            uint32_t tot;
+            const uint32_t pos = run + block_exclusive_scan(live ? 1u : 0u, &tot);
+            if (live) {
+#pragma unroll
+                for (int q = 0; q < NP; ++q) key2[q * cap + pos] = st.key[q * cap + i];
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk)
+                    CP::store(coef2 + (size_t)kk * CP::NS * cap, cap, pos, st.ld(st.coef, kk, i));
+            }
+            run += tot;
+        }
+    }
+}
+
+// Copy the compacted planes back, clear the index, then (next kernel) rebuild it.
+template <int NP, class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_cmp_copyback(Store<CP, K> st, const uint64_t* key2,
+                                                         const typename CP::store_t* coef2, Epochs ep,
+                                                         uint32_t e_prev, uint32_t e) {
+    if (run_stopped(ep)) return;
+    const uint32_t n0 = ep.n[e_prev] + ep.app[e_prev];
+    const uint32_t dead0 = (uint32_t)((int32_t)ep.dead[e_prev] + ep.dd[e_prev]);
+    if (!compact_wanted(n0, dead0)) return;
+    const uint32_t n = ep.n[e];
+    const size_t cap = st.cap;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += stride) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) st.key[q * cap + i] = key2[q * cap + i];
+#pragma unroll
+        for (int kk = 0; kk < K * CP::NS; ++kk) st.coef[kk * cap + i] = coef2[kk * cap + i];
+    }
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i <= st.hmask; i += stride) st.index[i] = kEmpty;
+}
+
+template <int NP>
+__global__ void __launch_bounds__(kBlock) k_index_build_async(const uint64_t* key, size_t cap, uint64_t* index,
+                                                              uint64_t hmask, Epochs ep, uint32_t e_prev,
+                                                              uint32_t e, int only_if_compacted, uint32_t* ierr) {
+    if (run_stopped(ep)) return;
+    if (only_if_compacted) {
+        const uint32_t n0 = ep.n[e_prev] + ep.app[e_prev];
+        const uint32_t dead0 = (uint32_t)((int32_t)ep.dead[e_prev] + ep.dd[e_prev]);
+        if (!compact_wanted(n0, dead0)) return;
+    }
+    const uint32_t n = ep.n[e];
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint64_t k[NP];
+#pragma unroll
+        for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+        index_insert(index, hmask, key_hash<NP>(k), i, ierr);
+    }
+}
+
+// Clifford block: remap keys in place (signs into the coefficients), clear the
+// index; k_index_build_async rebuilds it afterwards.
+template <int W, class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_cliff_async(Store<CP, K> st, const CliffGate* gates, int ng, Epochs ep,
+                                                        uint32_t e) {
+    if (run_stopped(ep)) return;
+    uint32_t n, dead;
+    epoch_in(ep, e, n, dead);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ep.n[e] = n;
+        ep.dead[e] = dead;
+    }
+    const size_t cap = st.cap;
+    const size_t stride = (size_t)gridDim.x * blockDim.x;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint64_t x[W], z[W];
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            x[w] = st.key[w * cap + i];
+            z[w] = st.key[(W + w) * cap + i];
+        }
+        int sgn = 1;
+        for (int t = ng - 1; t >= 0; --t) {
+            const CliffGate gt = gates[t];
+            const int a = gt.q0, b = gt.q1;
+            const bool xa = gbit(x, a), za = gbit(z, a);
+            switch (gt.kind) {
+                case 1:  // H: X <-> Z, Y -> -Y
+                    sbit(x, a, za);
+                    sbit(z, a, xa);
+                    if (xa && za) sgn = -sgn;
+                    break;
+                case 2:  // S: X -> -Y, Y -> X
+                    sbit(z, a, xa ^ za);
+                    if (xa && !za) sgn = -sgn;
+                    break;
+                case 3: {  // CNOT(a -> b)
+                    const bool xb = gbit(x, b), zb = gbit(z, b);
+                    sbit(x, b, xa ^ xb);
+                    sbit(z, a, za ^ zb);
+                    if (xa && zb && !(za ^ xb)) sgn = -sgn;
+                    break;
+                }
+                case 4: {  // SWAP
+                    const bool xb = gbit(x, b), zb = gbit(z, b);
+                    sbit(x, a, xb);
+                    sbit(x, b, xa);
+                    sbit(z, a, zb);
+                    sbit(z, b, za);
+                    break;
+                }
+                case 5: {  // CZ
+                    const bool xb = gbit(x, b), zb = gbit(z, b);
+                    if (xa && xb && (za ^ zb)) sgn = -sgn;
+                    sbit(z, a, za ^ xb);
+                    sbit(z, b, zb ^ xa);
+                    break;
+                }
+                default:
+                    break;
+            }
+        }
+#pragma unroll
+        for (int w = 0; w < W; ++w) {
+            st.key[w * cap + i] = x[w];
+            st.key[(W + w) * cap + i] = z[w];
+        }
+        if (sgn < 0 && !CP::dead(st.ld(st.coef, 0, i))) {
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) st.st(st.coef, kk, i, CP::neg(st.ld(st.coef, kk, i)));
+        }
+    }
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i <= st.hmask; i += stride) st.index[i] = kEmpty;
+}
+
+// Remove index entries of slots >= n (the appends of a step that overflowed).
+__global__ void k_index_rollback(uint64_t* index, size_t nslots, uint32_t n) {
+    for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nslots; i += (size_t)gridDim.x * blockDim.x) {
+        const uint64_t e = index[i];
+        if (e != kEmpty && (uint32_t)e >= n) index[i] = kEmpty;
+    }
+}
+
+}  // namespace ffx

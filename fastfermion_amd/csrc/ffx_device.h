@@ -26,16 +26,18 @@ constexpr uint32_t kDead32 = 0x7FA0DEADu;            // signalling-NaN payload
 // Per-observable control block in device memory.
 struct Ctrl {
     uint32_t n;           // slots in use (live + erased) at the start of the next step
-    uint32_t n_app;       // slots appended by the running step
+    uint32_t n_app;       // slots appended by the running step (step-by-step kernels)
     uint32_t n_dead;      // erased slots among [0, n)
     uint32_t done;        // finished blocks of the running kernel
     int32_t dead_delta;   // erased-count change of the running step
     uint32_t overflow;    // an append did not fit into cap
     uint32_t dup;         // duplicate keys found at upload
-    uint32_t pad;
-    unsigned long long updates;  // term-gate updates so far
+    uint32_t index_full;  // a probe or insert visited every index entry (internal error)
+    unsigned long long updates;  // term-gate updates so far (sum of live terms entering a rotation)
+    unsigned long long outs;     // sum of live terms leaving a rotation
     unsigned long long peak;     // largest live count after a rotation
     unsigned long long n_rot;    // rotations applied
+    unsigned long long n_compact;  // compactions run
 };
 
 __device__ __forceinline__ uint64_t mix64(uint64_t h) {
@@ -261,14 +263,21 @@ struct MajoranaAlg {
 // Find slot index of key r among slots [0, n).  Entries appended by the running
 // kernel have slot >= n and are skipped (their key words may not be visible yet;
 // a running step never looks up a key it appends itself).
+// The index always keeps at least half of its entries empty, so a probe ends
+// long before it has visited hmask + 1 entries; the bound only turns an internal
+// error into a reported one instead of an endless loop.
 template <int NP>
 __device__ __forceinline__ uint32_t probe(const uint64_t* __restrict__ key, size_t cap,
                                           const uint64_t* __restrict__ index, uint64_t hmask,
-                                          const uint64_t (&r)[NP], uint32_t n) {
+                                          const uint64_t (&r)[NP], uint32_t n, uint32_t* ierr = nullptr) {
     const uint64_t h = key_hash<NP>(r);
     const uint32_t tag = (uint32_t)(h >> 32);
     uint64_t p = h & hmask;
-    while (true) {
+    for (uint64_t step = 0;; ++step) {
+        if (step > hmask) {
+            if (ierr) atomicOr(ierr, 1u);
+            return kNone;
+        }
         const uint64_t e = index[p];
         if (e == kEmpty) return kNone;
         const uint32_t idx = (uint32_t)e;
@@ -286,12 +295,17 @@ __device__ __forceinline__ uint32_t probe(const uint64_t* __restrict__ key, size
     }
 }
 
-__device__ __forceinline__ void index_insert(uint64_t* index, uint64_t hmask, uint64_t h, uint32_t idx) {
+__device__ __forceinline__ void index_insert(uint64_t* index, uint64_t hmask, uint64_t h, uint32_t idx,
+                                             uint32_t* ierr = nullptr) {
     const uint64_t e = ((h >> 32) << 32) | (uint64_t)idx;
     uint64_t p = h & hmask;
-    while (atomicCAS((unsigned long long*)&index[p], (unsigned long long)kEmpty, (unsigned long long)e) !=
-           (unsigned long long)kEmpty)
+    for (uint64_t step = 0; step <= hmask; ++step) {
+        if (atomicCAS((unsigned long long*)&index[p], (unsigned long long)kEmpty, (unsigned long long)e) ==
+            (unsigned long long)kEmpty)
+            return;
         p = (p + 1) & hmask;
+    }
+    if (ierr) atomicOr(ierr, 1u);
 }
 
 __device__ __forceinline__ uint32_t atomic_load_u32(const uint32_t* p) {

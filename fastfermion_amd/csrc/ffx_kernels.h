@@ -18,6 +18,7 @@
 namespace ffx {
 
 constexpr int kBlock = 256;
+constexpr int kMaxWaves = 16;  // helpers below support blocks of up to 1024 threads
 constexpr int kItems = 4;
 constexpr int kTile = kBlock * kItems;
 
@@ -66,7 +67,7 @@ __device__ __forceinline__ uint64_t lanemask_lt() {
 
 // Block-wide sum of an int, result valid in thread 0.
 __device__ __forceinline__ int block_sum_int(int v) {
-    __shared__ int red[kBlock / 64];
+    __shared__ int red[kMaxWaves];
     for (int o = 32; o > 0; o >>= 1) v += __shfl_down(v, o, 64);
     const int wid = threadIdx.x >> 6;
     if (__lane_id() == 0) red[wid] = v;
@@ -80,7 +81,7 @@ __device__ __forceinline__ int block_sum_int(int v) {
 
 // Block-wide exclusive scan of a uint32; returns the prefix, total in *total.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
-    __shared__ uint32_t wsum[kBlock / 64];
+    __shared__ uint32_t wsum[kMaxWaves];
     const int lane = __lane_id();
     const int wid = threadIdx.x >> 6;
     uint32_t x = v;
@@ -118,6 +119,7 @@ __device__ __forceinline__ void fold_step(Ctrl* ctrl, int block_dead_delta, bool
             if (is_rotation) {
                 ctrl->updates += (unsigned long long)(n - dead_in);
                 const unsigned long long live = (unsigned long long)(n_out - dead_out);
+                ctrl->outs += live;
                 if (live > ctrl->peak) ctrl->peak = live;
                 ctrl->n_rot += 1;
             }
@@ -131,102 +133,137 @@ __device__ __forceinline__ void fold_step(Ctrl* ctrl, int block_dead_delta, bool
 }
 
 // ---------------------------------------------------------------------------
-// One rotation
+// One rotation: the per-slot work over [0, n), grid-stride from block `blk` of
+// `nblk`.  Reads coefficients from cin, writes every slot's new value to cout,
+// appends new partners at slots n + atomicAdd(app_ctr).  Returns this thread's
+// change of the erased-slot count.
 // ---------------------------------------------------------------------------
+// Per-slot work of one rotation on slot i < n: writes the slot's new
+// coefficient to cout and reports whether a new partner (key r, value vapp) must
+// be appended, plus the slot's change of the erased count.
 template <class Alg, class CP, int K>
-__global__ void __launch_bounds__(kBlock) k_rot(Store<CP, K> st, typename Alg::Gate g, RotParams<CP> rp) {
+__device__ __forceinline__ void rot_slot(const Store<CP, K>& st, const uint64_t* key,
+                                         const typename CP::store_t* __restrict__ cin,
+                                         typename CP::store_t* __restrict__ cout, const typename Alg::Gate& g,
+                                         const RotParams<CP>& rp, uint32_t n, uint32_t i, bool& need_app,
+                                         uint64_t (&r)[Alg::NP], typename CP::val_t (&vapp)[K], int& dead_delta) {
+    constexpr int NP = Alg::NP;
+    constexpr int NS = CP::NS;
+    using val_t = typename CP::val_t;
+    const size_t cap = st.cap;
+    uint64_t k[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+    val_t c[K];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) c[kk] = CP::load(cin + (size_t)kk * NS * cap, cap, i);
+    const bool alive = !CP::dead(c[0]);
+    val_t o[K];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) o[kk] = c[kk];
+    if (Alg::anticommutes(k, g)) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) r[q] = k[q] ^ g.p[q];
+        const uint32_t j = probe<NP>(key, cap, st.index, st.hmask, r, n, &st.ctrl->index_full);
+        val_t cj[K];
+        bool j_alive = false;
+        if (j != kNone) {
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) cj[kk] = CP::load(cin + (size_t)kk * NS * cap, cap, j);
+            j_alive = !CP::dead(cj[0]);
+        }
+        const bool recv = j_alive && Alg::degree(k) <= rp.maxdeg;
+        const int s_in = recv ? Alg::sign(r, k, g) : 1;
+        if (alive) {
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) o[kk] = CP::mul(c[kk], rp.cs);
+            if (recv) {
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) o[kk] = CP::add(o[kk], CP::mul_signed(cj[kk], rp.sn, s_in));
+            }
+            if (j == kNone && Alg::degree(r) <= rp.maxdeg) {
+                const int s_out = Alg::sign(k, r, g);
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) vapp[kk] = CP::mul_signed(c[kk], rp.sn, s_out);
+                need_app = true;
+                if (rp.trunc_thresh && magnitude<CP, K>(vapp) <= rp.mincoeff) need_app = false;
+                if (rp.trunc_xw && Alg::xweight(r) > rp.max_xw) need_app = false;
+            }
+        } else if (recv) {
+            // revive an erased slot in place (0 + v in the reference)
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) o[kk] = CP::mul_signed(cj[kk], rp.sn, s_in);
+        }
+    }
+    bool dead_out = CP::dead(o[0]);
+    if (!dead_out && rp.trunc_thresh && magnitude<CP, K>(o) <= rp.mincoeff) dead_out = true;
+    if (!dead_out && rp.trunc_xw && Alg::xweight(k) > rp.max_xw) dead_out = true;
+    if (dead_out) {
+        o[0] = CP::make_dead();
+#pragma unroll
+        for (int kk = 1; kk < K; ++kk) o[kk] = CP::from_double(0.0);
+    }
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) CP::store(cout + (size_t)kk * NS * cap, cap, i, o[kk]);
+    dead_delta += (alive && dead_out) ? 1 : 0;
+    dead_delta -= (!alive && !dead_out) ? 1 : 0;
+}
+
+// Store a new partner at `slot` and index it.
+template <int NP, class CP, int K>
+__device__ __forceinline__ void put_slot(const Store<CP, K>& st, uint64_t* key, typename CP::store_t* cout,
+                                         uint64_t slot, const uint64_t (&r)[NP],
+                                         const typename CP::val_t (&vapp)[K]) {
+    const size_t cap = st.cap;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) key[q * cap + slot] = r[q];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) CP::store(cout + (size_t)kk * CP::NS * cap, cap, slot, vapp[kk]);
+    index_insert(st.index, st.hmask, key_hash<NP>(r), (uint32_t)slot, &st.ctrl->index_full);
+}
+
+// Grid-stride sweep of one rotation with wave-aggregated appends.
+template <class Alg, class CP, int K>
+__device__ __forceinline__ int rot_sweep(const Store<CP, K>& st, const uint64_t* key,
+                                         const typename CP::store_t* __restrict__ cin,
+                                         typename CP::store_t* __restrict__ cout, const typename Alg::Gate& g,
+                                         const RotParams<CP>& rp, uint32_t n, uint32_t* app_ctr,
+                                         uint32_t* overflow, uint32_t blk, uint32_t nblk) {
     constexpr int NP = Alg::NP;
     using val_t = typename CP::val_t;
-    const uint32_t n = st.ctrl->n;
-    const size_t cap = st.cap;
     int dead_delta = 0;
-    const uint32_t stride = gridDim.x * blockDim.x;
-    for (uint32_t base = blockIdx.x * blockDim.x; base < n; base += stride) {
+    const uint32_t stride = nblk * blockDim.x;
+    for (uint32_t base = blk * blockDim.x; base < n; base += stride) {
         const uint32_t i = base + threadIdx.x;
-        const bool active = i < n;
         uint64_t r[NP];
         val_t vapp[K];
         bool need_app = false;
-        if (active) {
-            uint64_t k[NP];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
-            val_t c[K];
-#pragma unroll
-            for (int kk = 0; kk < K; ++kk) c[kk] = st.ld(st.coef, kk, i);
-            const bool alive = !CP::dead(c[0]);
-            val_t o[K];
-#pragma unroll
-            for (int kk = 0; kk < K; ++kk) o[kk] = c[kk];
-            if (Alg::anticommutes(k, g)) {
-#pragma unroll
-                for (int q = 0; q < NP; ++q) r[q] = k[q] ^ g.p[q];
-                const uint32_t j = probe<NP>(st.key, cap, st.index, st.hmask, r, n);
-                val_t cj[K];
-                bool j_alive = false;
-                if (j != kNone) {
-#pragma unroll
-                    for (int kk = 0; kk < K; ++kk) cj[kk] = st.ld(st.coef, kk, j);
-                    j_alive = !CP::dead(cj[0]);
-                }
-                const bool recv = j_alive && Alg::degree(k) <= rp.maxdeg;
-                const int s_in = recv ? Alg::sign(r, k, g) : 1;
-                if (alive) {
-#pragma unroll
-                    for (int kk = 0; kk < K; ++kk) o[kk] = CP::mul(c[kk], rp.cs);
-                    if (recv) {
-#pragma unroll
-                        for (int kk = 0; kk < K; ++kk) o[kk] = CP::add(o[kk], CP::mul_signed(cj[kk], rp.sn, s_in));
-                    }
-                    if (j == kNone && Alg::degree(r) <= rp.maxdeg) {
-                        const int s_out = Alg::sign(k, r, g);
-#pragma unroll
-                        for (int kk = 0; kk < K; ++kk) vapp[kk] = CP::mul_signed(c[kk], rp.sn, s_out);
-                        need_app = true;
-                        if (rp.trunc_thresh && magnitude<CP, K>(vapp) <= rp.mincoeff) need_app = false;
-                        if (rp.trunc_xw && Alg::xweight(r) > rp.max_xw) need_app = false;
-                    }
-                } else if (recv) {
-                    // revive an erased slot in place (0 + v in the reference)
-#pragma unroll
-                    for (int kk = 0; kk < K; ++kk) o[kk] = CP::mul_signed(cj[kk], rp.sn, s_in);
-                }
-            }
-            bool dead_out = CP::dead(o[0]);
-            if (!dead_out && rp.trunc_thresh && magnitude<CP, K>(o) <= rp.mincoeff) dead_out = true;
-            if (!dead_out && rp.trunc_xw && Alg::xweight(k) > rp.max_xw) dead_out = true;
-            if (dead_out) {
-                o[0] = CP::make_dead();
-#pragma unroll
-                for (int kk = 1; kk < K; ++kk) o[kk] = CP::from_double(0.0);
-            }
-#pragma unroll
-            for (int kk = 0; kk < K; ++kk) st.st(st.coef2, kk, i, o[kk]);
-            dead_delta += (alive && dead_out) ? 1 : 0;
-            dead_delta -= (!alive && !dead_out) ? 1 : 0;
-        }
-        // wave-aggregated append of new partner keys
+        if (i < n) rot_slot<Alg, CP, K>(st, key, cin, cout, g, rp, n, i, need_app, r, vapp, dead_delta);
         const uint64_t m = __ballot(need_app);
         if (m) {
             const int leader = __ffsll((unsigned long long)m) - 1;
             uint32_t first = 0;
-            if (__lane_id() == leader) first = atomicAdd(&st.ctrl->n_app, (uint32_t)__popcll(m));
+            if (__lane_id() == leader) first = atomicAdd(app_ctr, (uint32_t)__popcll(m));
             first = __shfl(first, leader, 64);
             if (need_app) {
                 const uint64_t slot = (uint64_t)n + first + __popcll(m & lanemask_lt());
-                if (slot < cap) {
-#pragma unroll
-                    for (int q = 0; q < NP; ++q) st.key[q * cap + slot] = r[q];
-#pragma unroll
-                    for (int kk = 0; kk < K; ++kk) st.st(st.coef2, kk, slot, vapp[kk]);
-                    index_insert(st.index, st.hmask, key_hash<NP>(r), (uint32_t)slot);
-                } else {
-                    atomicOr(&st.ctrl->overflow, 1u);
-                }
+                if (slot < st.cap)
+                    put_slot<NP, CP, K>(st, const_cast<uint64_t*>(key), cout, slot, r, vapp);
+                else
+                    atomicOr(overflow, 1u);
             }
         }
     }
-    fold_step(st.ctrl, block_sum_int(dead_delta), true);
+    return dead_delta;
+}
+
+// One rotation as its own launch (step-by-step engine).
+template <class Alg, class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_rot(Store<CP, K> st, typename Alg::Gate g, RotParams<CP> rp) {
+    const uint32_t n = st.ctrl->n;
+    const int dd = rot_sweep<Alg, CP, K>(st, st.key, st.coef, st.coef2, g, rp, n, &st.ctrl->n_app,
+                                         &st.ctrl->overflow, blockIdx.x, gridDim.x);
+    fold_step(st.ctrl, block_sum_int(dd), true);
 }
 
 // ---------------------------------------------------------------------------
@@ -322,13 +359,12 @@ __global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
 
 template <int NP>
 __global__ void __launch_bounds__(kBlock) k_index_build(const uint64_t* key, size_t cap, uint64_t* index,
-                                                        uint64_t hmask, const Ctrl* ctrl) {
-    const uint32_t n = ctrl->n;
+                                                        uint64_t hmask, Ctrl* ctrl, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t k[NP];
 #pragma unroll
         for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
-        index_insert(index, hmask, key_hash<NP>(k), i);
+        index_insert(index, hmask, key_hash<NP>(k), i, &ctrl->index_full);
     }
 }
 
@@ -341,7 +377,7 @@ __global__ void __launch_bounds__(kBlock) k_index_check(const uint64_t* key, siz
         uint64_t k[NP];
 #pragma unroll
         for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
-        if (probe<NP>(key, cap, index, hmask, k, n) != i) atomicOr(&ctrl->dup, 1u);
+        if (probe<NP>(key, cap, index, hmask, k, n, &ctrl->index_full) != i) atomicOr(&ctrl->dup, 1u);
     }
 }
 

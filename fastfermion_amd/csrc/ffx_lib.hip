@@ -19,7 +19,7 @@
 #include <vector>
 
 #include "../../include/fastfermion_amd.h"
-#include "ffx_kernels.h"
+#include "ffx_async.h"
 
 using namespace ffx;
 
@@ -48,6 +48,12 @@ struct ffx_ctx {
     size_t staging_bytes = 0;
     CliffGate* cliff = nullptr;
     size_t cliff_cap = 0;
+    uint32_t* epochs = nullptr;
+    size_t epochs_cap = 0;
+    int n_cu = 0;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    int kernel_timing = 0;
+    std::vector<hipEvent_t> kev;  // event pairs around rotation launches
     // description of the device-resident result
     int last_kind = 0;  // 0 none, 1 Pauli, 2 Majorana
     int last_W = 0, last_K = 0, last_prec = 0;
@@ -220,6 +226,11 @@ std::vector<Step> pauli_schedule(const ffx_pauli_gate* g, int64_t n, bool batche
     return steps;
 }
 
+// Gate factors: one glibc sincos() call per angle, which is what GCC -O2 makes of
+// the reference's cos(theta); sin(theta) pair (src/pauli/propagate.h:76-77,
+// src/majorana_gates.h:86-88).  sincos and a separate sin can differ by one ulp.
+inline void gate_trig(double theta, double& c, double& s) { ::sincos(theta, &s, &c); }
+
 template <class CP>
 typename CP::fac_t make_fac(double v);
 template <>
@@ -268,12 +279,17 @@ struct Engine {
         return FFX_OK;
     }
 
+    // Index over slots [0, n_host).  The control block's n / n_dead are set from
+    // the host values first: every later kernel of the step-wise paths reads them.
     int rebuild_index() {
+        if (n_host > ctx->cap) return set_err(ctx, FFX_EHIP, "internal error: slot count exceeds capacity");
+        HIPCHK(hipMemcpyAsync(&ctx->ctrl->n, &n_host, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(&ctx->ctrl->n_dead, &dead_host, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
         hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx->hslots, kBlock, 8192)), dim3(kBlock), 0, ctx->stream,
                            ctx->index, ctx->hslots, kEmpty);
         CHK(launch_check());
         hipLaunchKernelGGL(k_index_build<NP>, dim3(grid_for(n_host, kBlock, 16384)), dim3(kBlock), 0, ctx->stream,
-                           ctx->key, ctx->cap, ctx->index, ctx->hslots - 1, ctx->ctrl);
+                           ctx->key, ctx->cap, ctx->index, ctx->hslots - 1, ctx->ctrl, n_host);
         return launch_check();
     }
 
@@ -339,6 +355,7 @@ struct Engine {
         n_host = ctx->ctrl_host->n;
         dead_host = ctx->ctrl_host->n_dead;
         if (ctx->ctrl_host->overflow) return set_err(ctx, FFX_EHIP, "internal error: term store overflow");
+        if (ctx->ctrl_host->index_full) return set_err(ctx, FFX_EHIP, "internal error: hash index full");
         return FFX_OK;
     }
 
@@ -508,12 +525,231 @@ struct Engine {
         return FFX_OK;
     }
 
+    // ---- asynchronous engine: one launch per step, no host sync between steps ----
+    struct AStep {
+        int kind = 0;  // 0 rotation, 1 Clifford block
+        typename Alg::Gate g{};
+        double cs = 1.0, sn = 0.0;
+        int trunc_thresh = 0, trunc_xw = 0;
+        int compact_after = 0;  // run the (device-decided) compaction sequence after this step
+        int cb = 0, cn = 0;     // Clifford gates [cb, cb+cn) of the block list
+    };
+    uint32_t* ep_buf = nullptr;  // owned by ctx (ctx->epochs)
+    int64_t st_updates = 0, st_outs = 0, st_peak = 0, st_rot = 0, st_compact = 0;
+
+    int ensure_epochs(size_t ne) {
+        if (ne <= ctx->epochs_cap) return FFX_OK;
+        dev_free(ctx->epochs);
+        ctx->epochs_cap = 0;
+        const size_t c = pow2ceil(ne);
+        CHK(dev_alloc(ctx, &ctx->epochs, (4 * c + 64) * sizeof(uint32_t)));
+        ctx->epochs_cap = c;
+        return FFX_OK;
+    }
+
+    Epochs epochs() const {
+        Epochs ep;
+        const size_t c = ctx->epochs_cap;
+        ep.n = ctx->epochs;
+        ep.dead = ctx->epochs + c;
+        ep.app = ctx->epochs + 2 * c;
+        ep.dd = (int32_t*)(ctx->epochs + 3 * c);
+        ep.flags = ctx->epochs + 4 * c;
+        return ep;
+    }
+
+    int run_async(const std::vector<AStep>& steps, const std::vector<CliffGate>& cliff, double mincoeff, int maxdeg,
+                  int max_xw) {
+        if (steps.empty()) return FFX_OK;
+        if (!cliff.empty()) {
+            if (cliff.size() > ctx->cliff_cap) {
+                dev_free(ctx->cliff);
+                CHK(dev_alloc(ctx, &ctx->cliff, cliff.size() * sizeof(CliffGate)));
+                ctx->cliff_cap = cliff.size();
+            }
+            HIPCHK(hipMemcpyAsync(ctx->cliff, cliff.data(), cliff.size() * sizeof(CliffGate), hipMemcpyHostToDevice,
+                                  ctx->stream));
+        }
+        if (ctx->n_cu <= 0) {
+            hipDeviceProp_t prop;
+            HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+            ctx->n_cu = prop.multiProcessorCount;
+        }
+        int rot_grid = 2 * ctx->n_cu;
+        if (const char* ev = getenv("FFX_ROT_GRID")) rot_grid = std::max(1, atoi(ev));
+        size_t s = 0;
+        for (int attempt = 0; attempt < 64 && s < steps.size(); ++attempt) {
+            // epochs needed from step s on
+            size_t ne = 1;
+            for (size_t t = s; t < steps.size(); ++t) ne += 1 + (steps[t].compact_after ? 1 : 0);
+            CHK(ensure_epochs(ne));
+            CHK(ensure_counts(ctx->cap / kTile + 2));
+            const Epochs ep = epochs();
+            HIPCHK(hipMemsetAsync(ctx->epochs, 0, (4 * ctx->epochs_cap + 64) * sizeof(uint32_t), ctx->stream));
+            uint32_t init[2] = {n_host, dead_host};
+            HIPCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+            std::vector<int64_t> step_of(ne, -1);
+            std::vector<int> is_rot(ne, 0);
+            std::vector<std::pair<void*, void*>> planes_at(steps.size());
+            const unsigned g_cap = grid_for(ctx->cap, kTile, 4096);
+            const unsigned g_idx = grid_for(std::max(ctx->cap, ctx->hslots), kBlock, 2048);
+            uint32_t e = 1;
+            size_t nev = 0;
+            if (ctx->kernel_timing) {
+                size_t nrot = 0;
+                for (size_t t = s; t < steps.size(); ++t) nrot += steps[t].kind == 0;
+                while (ctx->kev.size() < 2 * nrot) {
+                    hipEvent_t ev;
+                    HIPCHK(hipEventCreate(&ev));
+                    ctx->kev.push_back(ev);
+                }
+            }
+            for (size_t t = s; t < steps.size(); ++t) {
+                const AStep& a = steps[t];
+                planes_at[t] = {ctx->coef, ctx->coef2};
+                if (a.kind == 0) {
+                    if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->kev[2 * nev], ctx->stream));
+                    RotParams<CP> rp{};
+                    rp.cs = make_fac<CP>(a.cs);
+                    rp.sn = make_fac<CP>(a.sn);
+                    rp.mincoeff = mincoeff;
+                    rp.maxdeg = maxdeg;
+                    rp.trunc_thresh = a.trunc_thresh;
+                    rp.trunc_xw = a.trunc_xw;
+                    rp.max_xw = max_xw;
+                    hipLaunchKernelGGL((k_rot_async<Alg, CP, K>), dim3(rot_grid), dim3(kBlockRot), 0, ctx->stream,
+                                       store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, a.g, rp, ep, e);
+                    CHK(launch_check());
+                    if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->kev[2 * nev++ + 1], ctx->stream));
+                    std::swap(ctx->coef, ctx->coef2);
+                    step_of[e] = (int64_t)t;
+                    is_rot[e] = 1;
+                    ++e;
+                } else {
+                    if constexpr (Alg::kPauli) {
+                        hipLaunchKernelGGL((k_cliff_async<Alg::W, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream,
+                                           store(), (const CliffGate*)(ctx->cliff + a.cb), a.cn, ep, e);
+                        CHK(launch_check());
+                        hipLaunchKernelGGL(k_index_build_async<NP>, dim3(g_idx), dim3(kBlock), 0, ctx->stream, ctx->key,
+                                           ctx->cap, ctx->index, ctx->hslots - 1, ep, e, e, 0,
+                                           &ctx->ctrl->index_full);
+                        CHK(launch_check());
+                    }
+                    step_of[e] = (int64_t)t;
+                    ++e;
+                }
+                if (a.compact_after) {
+                    const uint32_t ep_prev = e - 1;
+                    hipLaunchKernelGGL((k_cmp_count<CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream, store(), ep,
+                                       ep_prev, ctx->counts);
+                    CHK(launch_check());
+                    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(kBlock), 0, ctx->stream, ep, ep_prev, ctx->counts);
+                    CHK(launch_check());
+                    hipLaunchKernelGGL((k_cmp_scatter<NP, CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream, store(),
+                                       ctx->key2, (store_t*)ctx->coef2, ep, ep_prev, e, ctx->counts);
+                    CHK(launch_check());
+                    hipLaunchKernelGGL((k_cmp_copyback<NP, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream, store(),
+                                       ctx->key2, (const store_t*)ctx->coef2, ep, ep_prev, e);
+                    CHK(launch_check());
+                    hipLaunchKernelGGL(k_index_build_async<NP>, dim3(g_idx), dim3(kBlock), 0, ctx->stream, ctx->key,
+                                       ctx->cap, ctx->index, ctx->hslots - 1, ep, ep_prev, e, 1,
+                                       &ctx->ctrl->index_full);
+                    CHK(launch_check());
+                    step_of[e] = (int64_t)t;
+                    ++e;
+                }
+            }
+            // one sync per run: epochs, flags
+            std::vector<uint32_t> hn(e), hd(e), ha(e);
+            std::vector<int32_t> hdd(e);
+            uint32_t hf[2] = {0, 0};
+            HIPCHK(hipMemcpyAsync(hn.data(), ep.n, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipMemcpyAsync(hd.data(), ep.dead, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipMemcpyAsync(ha.data(), ep.app, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipMemcpyAsync(hdd.data(), ep.dd, e * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipMemcpyAsync(hf, ep.flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+            HIPCHK(hipStreamSynchronize(ctx->stream));
+            for (size_t q = 0; q < nev; ++q) {
+                float ms = 0.f;
+                HIPCHK(hipEventElapsedTime(&ms, ctx->kev[2 * q], ctx->kev[2 * q + 1]));
+                ctx->prof.seconds_kernel += 1e-3 * ms;
+                ctx->prof.run_launches += 1;
+            }
+            const uint32_t e_end = hf[0] ? hf[1] : e;
+            for (uint32_t q = 1; q < e_end; ++q) {
+                if (!is_rot[q]) {
+                    if (hn[q] != hn[q - 1] + ha[q - 1]) st_compact++;
+                    continue;
+                }
+                st_rot++;
+                st_updates += (int64_t)(hn[q] - hd[q]);
+                const int64_t live = (int64_t)(hn[q] + ha[q]) - (int64_t)((int32_t)hd[q] + hdd[q]);
+                st_outs += live;
+                st_peak = std::max<int64_t>(st_peak, live);
+            }
+            if (!hf[0]) {
+                n_host = hn[e - 1] + ha[e - 1];
+                dead_host = (uint32_t)((int32_t)hd[e - 1] + hdd[e - 1]);
+                s = steps.size();
+                break;
+            }
+            // overflow at epoch hf[1]: restore the state at the start of that step and grow
+            const uint32_t ef = hf[1];
+            const size_t tf = (size_t)step_of[ef];
+            ctx->coef = planes_at[tf].first;
+            ctx->coef2 = planes_at[tf].second;
+            n_host = hn[ef];
+            dead_host = hd[ef];
+            {
+                Ctrl c{};
+                c.n = n_host;
+                c.n_dead = dead_host;
+                HIPCHK(hipMemcpyAsync(ctx->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+            }
+            hipLaunchKernelGGL(k_index_rollback, dim3(grid_for(ctx->hslots, kBlock, 8192)), dim3(kBlock), 0, ctx->stream,
+                               ctx->index, ctx->hslots, n_host);
+            CHK(launch_check());
+            CHK(reserve(2 * ctx->cap));
+            s = tf;
+        }
+        if (s < steps.size()) return set_err(ctx, FFX_ECAPACITY, "term store kept overflowing");
+        HIPCHK(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->ctrl_host->index_full) return set_err(ctx, FFX_EHIP, "internal error: hash index full");
+        Ctrl c{};
+        c.n = n_host;
+        c.n_dead = dead_host;
+        HIPCHK(hipMemcpyAsync(ctx->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        *ctx->ctrl_host = c;
+        return FFX_OK;
+    }
+
     void finish_profile() {
-        ctx->prof.n_gates = (int64_t)ctx->ctrl_host->n_rot;
-        ctx->prof.updates = (int64_t)ctx->ctrl_host->updates;
-        ctx->prof.peak_terms = (int64_t)ctx->ctrl_host->peak;
+        if (st_rot) {
+            ctx->prof.n_gates = st_rot;
+            ctx->prof.updates = st_updates;
+            ctx->prof.outs = st_outs;
+            ctx->prof.peak_terms = st_peak;
+            ctx->prof.compactions = st_compact;
+        } else {
+            ctx->prof.n_gates = (int64_t)ctx->ctrl_host->n_rot;
+            ctx->prof.updates = (int64_t)ctx->ctrl_host->updates;
+            ctx->prof.outs = (int64_t)ctx->ctrl_host->outs;
+            ctx->prof.peak_terms = (int64_t)ctx->ctrl_host->peak;
+            ctx->prof.compactions = (int64_t)ctx->ctrl_host->n_compact;
+        }
     }
 };
+
+// FFX_ENGINE=steps selects the one-launch-per-rotation engine with a host sync
+// after every rotation (kept as a cross-check); the default is the asynchronous
+// engine (one launch per step, one sync per propagation).
+bool step_engine() {
+    const char* e = getenv("FFX_ENGINE");
+    return e && std::strcmp(e, "steps") == 0;
+}
 
 template <int W>
 typename PauliAlg<W>::Gate pauli_gate_dev(const ffx_pauli_gate& g) {
@@ -558,12 +794,45 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
     const double t1 = now_s();
     const std::vector<Step> steps = pauli_schedule<W>(gates, n_gates, o.batched != 0);
     const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
+    if (o.topk > 0) return set_err(ctx, FFX_EINVAL, "topk truncation is not supported on the device path");
+    if (!step_engine()) {
+        using AStep = typename Engine<Alg, CP, K>::AStep;
+        std::vector<AStep> sd;
+        std::vector<CliffGate> cl;
+        sd.reserve(steps.size());
+        int trunc_points = 0;
+        for (const Step& s : steps) {
+            AStep d{};
+            if (s.kind == 0) {
+                const ffx_pauli_gate& g = gates[s.gate];
+                d.g = pauli_gate_dev<W>(g);
+                gate_trig(g.theta, d.cs, d.sn);
+                d.kind = 0;
+                d.trunc_thresh = (s.trunc && o.mincoeff > 0) ? 1 : 0;
+                d.trunc_xw = (s.trunc && o.max_xweight >= 0 && o.xtrunc_period > 0 &&
+                              (s.rc + 1) % o.xtrunc_period == 0)
+                                 ? 1
+                                 : 0;
+                // batched runs compact at every batch end; per-gate runs every 16th truncation
+                if (s.trunc) d.compact_after = (o.batched || (++trunc_points % 16) == 0) ? 1 : 0;
+            } else {
+                d.kind = 1;
+                d.cb = (int)cl.size();
+                d.cn = (int)(s.ce - s.cb);
+                for (int64_t t = s.cb; t < s.ce; ++t) cl.push_back(CliffGate{gates[t].kind, gates[t].q0, gates[t].q1, 0});
+            }
+            sd.push_back(d);
+        }
+        CHK(eng.run_async(sd, cl, o.mincoeff, maxdeg, o.max_xweight));
+    } else
     for (const Step& s : steps) {
         if (s.kind == 0) {
             const ffx_pauli_gate& g = gates[s.gate];
             RotParams<CP> rp{};
-            rp.cs = make_fac<CP>(std::cos(g.theta));
-            rp.sn = make_fac<CP>(std::sin(g.theta));
+            double cs_, sn_;
+            gate_trig(g.theta, cs_, sn_);
+            rp.cs = make_fac<CP>(cs_);
+            rp.sn = make_fac<CP>(sn_);
             rp.maxdeg = maxdeg;
             rp.mincoeff = o.mincoeff;
             rp.trunc_thresh = (s.trunc && o.mincoeff > 0) ? 1 : 0;
@@ -607,11 +876,29 @@ int run_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, 
     CHK(eng.upload(m, nullptr, re, im, n_terms));
     const double t1 = now_s();
     const int maxdeg = maxdegree < 0 ? (1 << 30) : maxdegree;
+    if (!step_engine()) {
+        using AStep = typename Engine<Alg, CP, K>::AStep;
+        std::vector<AStep> sd;
+        sd.reserve((size_t)n_gates);
+        int cnt = 0;
+        for (int64_t i = n_gates - 1; i >= 0; --i) {
+            AStep d{};
+            d.g = majorana_gate_dev<W>(gates[i]);
+            gate_trig(gates[i].theta, d.cs, d.sn);
+            d.kind = 0;
+            d.trunc_thresh = mincoeff > 0 ? 1 : 0;
+            d.compact_after = (mincoeff > 0 && (++cnt % 16) == 0) ? 1 : 0;
+            sd.push_back(d);
+        }
+        CHK(eng.run_async(sd, {}, mincoeff, maxdeg, -1));
+    } else
     for (int64_t i = n_gates - 1; i >= 0; --i) {
         const ffx_majorana_gate& g = gates[i];
         RotParams<CP> rp{};
-        rp.cs = make_fac<CP>(std::cos(g.theta));
-        rp.sn = make_fac<CP>(std::sin(g.theta));
+        double cs_, sn_;
+        gate_trig(g.theta, cs_, sn_);
+        rp.cs = make_fac<CP>(cs_);
+        rp.sn = make_fac<CP>(sn_);
         rp.maxdeg = maxdeg;
         rp.mincoeff = mincoeff;
         rp.trunc_thresh = mincoeff > 0 ? 1 : 0;
@@ -686,6 +973,7 @@ int ffx_ctx_create(int device, ffx_ctx** out) {
     if (hipHostMalloc((void**)&ctx->result_host, 2 * sizeof(double), hipHostMallocDefault) != hipSuccess)
         return fail(FFX_ENOMEM);
     std::memset(ctx->ctrl_host, 0, sizeof(Ctrl));
+    if (hipEventCreate(&ctx->ev0) != hipSuccess || hipEventCreate(&ctx->ev1) != hipSuccess) return fail(FFX_EHIP);
     *out = ctx;
     return FFX_OK;
 }
@@ -702,6 +990,10 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     dev_free(ctx->result);
     dev_free(ctx->staging);
     dev_free(ctx->cliff);
+    dev_free(ctx->epochs);
+    for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
+    if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
+    if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
     if (ctx->ctrl_host) (void)hipHostFree(ctx->ctrl_host);
     if (ctx->result_host) (void)hipHostFree(ctx->result_host);
     if (ctx->stream) (void)hipStreamDestroy(ctx->stream);
@@ -720,6 +1012,12 @@ void ffx_terms_free(ffx_terms* t) {
     t->x = t->z = nullptr;
     t->c_re = t->c_im = nullptr;
     t->n_terms = 0;
+}
+
+int ffx_set_kernel_timing(ffx_ctx* ctx, int on) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->kernel_timing = on ? 1 : 0;
+    return FFX_OK;
 }
 
 int ffx_profile_get(const ffx_ctx* ctx, ffx_profile* out) {

@@ -112,6 +112,10 @@ typedef struct ffx_profile {
     double seconds_upload;
     double seconds_propagate;
     double seconds_download;
+    int64_t outs;            /* sum of live terms leaving each rotation */
+    int64_t compactions;     /* erased-slot compactions run */
+    int64_t run_launches;    /* launches of the propagation kernel proper */
+    double seconds_kernel;   /* HIP-event time of those launches */
 } ffx_profile;
 
 typedef struct ffx_ctx ffx_ctx;
@@ -155,6 +159,11 @@ void ffx_terms_free(ffx_terms* t);
 /* Counters of the last propagate call (the reference's profile_get dictionary,
  * src/python_bind.cpp:427-446, reports CPU phases; these are device phases). */
 int ffx_profile_get(const ffx_ctx* ctx, ffx_profile* out);
+
+/* Kernel timing: when on, every propagation kernel launch is bracketed by a
+ * pair of HIP events on the context's stream and ffx_profile.seconds_kernel /
+ * run_launches report their summed time and count (no reference counterpart). */
+int ffx_set_kernel_timing(ffx_ctx* ctx, int on);
 
 /* Number of visible HIP devices (0 when none); never fails. */
 int ffx_device_count(void);

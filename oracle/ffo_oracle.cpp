@@ -194,6 +194,13 @@ class DenseMap {
     }
 };
 
+// Gate factors cos(theta), sin(theta).  The reference evaluates cos(theta) and
+// sin(theta) of one angle in one function (src/pauli/propagate.h:76-77,
+// src/majorana_gates.h:86-88); GCC at -O2 folds such a pair into one glibc
+// sincos() call, whose sin can differ by one ulp from a separate sin() call on
+// FMA-capable x86-64.  Both the oracle and the device library call sincos().
+inline void gate_trig(double theta, double& c, double& s) { ::sincos(theta, &s, &c); }
+
 // i^m (src/common.h:57-66)
 inline int m1pow(int m) { return (m % 2 == 0) ? 1 : -1; }
 inline cplx Ipow(int m) {
@@ -349,8 +356,9 @@ template <int W>
 void conjugate(PMap<W>& obs, const Rot<W>& gate, int maxdegree) {
     std::vector<std::pair<PKey<W>, cplx>> new_terms;
     new_terms.reserve(obs.size());
-    const double cos_t = cos(gate.theta);
-    const cplx isin_t = cplx(0, sin(gate.theta));
+    double cos_t, sin_t;
+    gate_trig(gate.theta, cos_t, sin_t);
+    const cplx isin_t = cplx(0, sin_t);
     for (size_t t = 0; t < obs.size(); ++t) {
         const PKey<W>& x = obs.keys[t];
         if (!commutes<W>(x, gate.p)) {
@@ -443,8 +451,9 @@ PMap<W> from_sharded(const Sharded<W>& s) {
 
 template <int W>
 void conjugate_sharded(Sharded<W>& shards, const Rot<W>& gate, int maxdegree, int n_threads) {
-    const double cos_t = cos(gate.theta);
-    const cplx isin_t = cplx(0, sin(gate.theta));
+    double cos_t, sin_t;
+    gate_trig(gate.theta, cos_t, sin_t);
+    const cplx isin_t = cplx(0, sin_t);
     using SendBuf = std::vector<std::pair<PKey<W>, cplx>>;
     std::vector<std::vector<SendBuf>> outgoing(n_threads, std::vector<SendBuf>(n_threads));
 #pragma omp parallel num_threads(n_threads)
@@ -513,8 +522,9 @@ void conjugate_omp(PMap<W>& obs, const Rot<W>& gate, int maxdegree, int n_thread
     std::vector<std::pair<PKey<W>, cplx>> snap;
     snap.reserve(obs.size());
     for (size_t t = 0; t < obs.size(); ++t) snap.emplace_back(obs.keys[t], obs.vals[t]);
-    const double cos_t = cos(gate.theta);
-    const cplx isin_t = cplx(0, sin(gate.theta));
+    double cos_t, sin_t;
+    gate_trig(gate.theta, cos_t, sin_t);
+    const cplx isin_t = cplx(0, sin_t);
     std::vector<std::vector<std::pair<PKey<W>, cplx>>> kept(n_threads), partners(n_threads);
 #pragma omp parallel num_threads(n_threads)
     {
@@ -693,8 +703,9 @@ MMap<W> majorana_propagate(const ffo_majorana_gate* circuit, int64_t n, MMap<W> 
         for (int w = 0; w < W; ++w) ms.m[w] = g.m[w];
         std::vector<std::pair<MKey<W>, cplx>> o_new;
         o_new.reserve(ret.size());
-        const double costheta = cos(g.theta);
-        const cplx iirsintheta = g.r ? cplx(-sin(g.theta), 0) : cplx(0, sin(g.theta));
+        double costheta, sintheta;
+        gate_trig(g.theta, costheta, sintheta);
+        const cplx iirsintheta = g.r ? cplx(-sintheta, 0) : cplx(0, sintheta);
         for (size_t t = 0; t < ret.size(); ++t) {
             const MKey<W>& x = ret.keys[t];
             if (!mcommutes<W>(x, ms)) {

@@ -256,3 +256,55 @@ def test_majorana_fock_expectation(ctx):
         _, ex = ctx.majorana_propagate(g, 1, m, c, maxdegree=6, mincoeff=1e-6, occ=[occ])
         want = pyffo.majorana_expect(ref, [occ])
         assert abs(ex - want) < 1e-12, (occ, ex, want)
+
+
+def test_capacity_growth_and_resume():
+    """A fresh context starts with a small store; the 6x6 kicked-Ising run
+    (peak ~1.4e6 live terms) overflows it several times and must resume each
+    time from the overflowing rotation with an identical result."""
+    g, _ = wl.kicked_ising(6, 4)
+    x, z, c = wl.single_pauli("Z", [21], 36)
+    ref = pyffo.pauli_propagate(g, 1, x, z, c, maxdegree=8, mincoeff=1e-5, batched=True)
+    fresh = Context(0)
+    try:
+        dev = fresh.pauli_propagate(g, 1, x, z, c, maxdegree=8, mincoeff=1e-5, batched=True)
+        assert_same_terms(dev, ref, exact=True)
+        assert fresh.profile()["peak_terms"] <= ref.peak_terms
+        assert fresh.profile()["updates"] == ref.updates
+    finally:
+        fresh.close()
+
+
+def test_step_engine_matches_async_engine(ctx, monkeypatch):
+    """The cross-check engine (host sync after every rotation) and the default
+    asynchronous engine produce identical terms and work counters."""
+    g, _ = wl.kicked_ising(5, 4)
+    x, z, c = wl.single_pauli("Z", [12], 25)
+    kw = dict(maxdegree=8, mincoeff=1e-6, batched=False)
+    a = ctx.pauli_propagate(g, 1, x, z, c, **kw)
+    pa = ctx.profile()
+    monkeypatch.setenv("FFX_ENGINE", "steps")
+    b = ctx.pauli_propagate(g, 1, x, z, c, **kw)
+    pb = ctx.profile()
+    assert_same_terms(a, b, exact=True)
+    assert pa["updates"] == pb["updates"] and pa["peak_terms"] == pb["peak_terms"]
+
+
+def test_layout_switch_after_large_result():
+    """Regression: a Majorana run leaving many slots, then a Pauli run with a
+    different store layout on the same context; the store is re-laid out and
+    the index rebuilt from the new slot count only."""
+    c2 = Context(0)
+    try:
+        rng = np.random.default_rng(5)
+        gm = random_majorana(rng, 16, 40, maxdeg=4)
+        m, cm = mobs([([0, 1], 0.5j), ([2, 3, 4, 5], 0.3)], 16)
+        big = c2.majorana_propagate(gm, 1, m, cm, maxdegree=8)
+        assert len(big) > 2048
+        x, z, c = obs([("Z", [0], 1.0)], 6)
+        g = np.concatenate([tfim(6, 0.1)] * 3)
+        ref = pyffo.pauli_propagate(g, 1, x, z, c)
+        dev = c2.pauli_propagate(g, 1, x, z, c)
+        assert_same_terms(dev, ref, exact=True)
+    finally:
+        c2.close()
