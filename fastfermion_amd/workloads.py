@@ -94,3 +94,28 @@ def pack_majorana(gates):
         out[i]["theta"] = theta
         out[i]["r"] = r
     return out
+
+
+def read_circuit(path: str):
+    """Gate table from the text circuit format of fastfermion's examples
+    (examples/common.py:3-35): first line the gate count, then one gate per
+    line: `ROT <axis letters> <qubits...> <theta>`, or `H i`, `S i`, `CNOT i j`,
+    `SWAP i j`, `CZ i j`."""
+    from ._abi import CNOT, CZ, H, S, SWAP, clifford_record, rot_record
+
+    kinds = {"H": H, "S": S, "CNOT": CNOT, "SWAP": SWAP, "CZ": CZ}
+    recs = []
+    with open(path, "r", encoding="utf-8") as f:
+        n = int(f.readline())
+        for line in f:
+            parts = line.split()
+            if not parts:
+                continue
+            if parts[0] == "ROT":
+                recs.append(rot_record(parts[1], [int(q) for q in parts[2:-1]], float(parts[-1])))
+            else:
+                qs = [int(q) for q in parts[1:]]
+                recs.append(clifford_record(kinds[parts[0]], qs[0], qs[1] if len(qs) > 1 else 0))
+    if len(recs) != n:
+        raise ValueError(f"{path}: header says {n} gates, found {len(recs)}")
+    return np.concatenate(recs) if recs else np.zeros(0, PAULI_GATE_DTYPE)

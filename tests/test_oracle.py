@@ -1,0 +1,327 @@
+"""Pins the CPU oracle (oracle/libffo_oracle.so) -- CPU only, no GPU.
+
+The reference C++ cannot be built here (it needs a <format> header this
+toolchain lacks), so every oracle function is pinned by checks that do not
+depend on the oracle's own formulas:
+
+  function (reference lines it restates)           pinned by
+  ---------------------------------------------------------------------------
+  commutes / multiply / phase / conjugate           single-rotation closed form U^dag Q U vs
+  (algebra.h:209-233, propagate.h:71-89)            dense matrices, all 2-qubit P, Q pairs
+  reverse walk + Clifford blocks + batches          dense U^dag O U decomposed in the Pauli
+  (propagate.h:272-493, gates.h:15-69,273-301)      basis for random circuits, n <= 4
+  truncate_all (truncate.h:16-57)                   hand-built cases: partner-only degree
+                                                    filter, |c| == mincoeff erased, exact
+                                                    zeros kept, X/Y-weight period, top-k
+  sharded / omp strategies (propagate.h:97-264)     equal to serial (reference omp/sharded suites)
+  product-state readout (qubitproductstate.h:73-88) dense <b|O|b>
+  whole Pauli path                                  reference golden su4circuit-16:
+                                                    <Z15> = 0.008664 +- 1e-6 (maxdegree=4,
+                                                    examples/pauli_propagation.py:5-11)
+  Majorana conjugation + sign (majorana_gates.h,    dense Jordan-Wigner matrices: single MROT
+  bits_utils.h:14-38, majorana_algebra.h:441-444)   closed form and random circuits, 4 modes
+  Fock readout (fockstate.h:95-114)                 dense <occ|O|occ>
+"""
+from __future__ import annotations
+
+import itertools
+import math
+import os
+
+import numpy as np
+import pytest
+
+import _dense as D
+import pyffo
+from fastfermion_amd import _abi, workloads as wl
+
+GOLDEN = os.path.join(os.path.dirname(__file__), "golden")
+
+
+def as_dict(res):
+    return {(int(res.x[i, 0]), int(res.z[i, 0])): res.c[i] for i in range(len(res.c))}
+
+
+def run1(gates, spec, **kw):
+    """Oracle on one-word keys; spec = {(x, z): coeff}."""
+    keys = list(spec)
+    x = np.array([[k[0]] for k in keys], np.uint64).reshape(-1, 1)
+    z = np.array([[k[1]] for k in keys], np.uint64).reshape(-1, 1)
+    c = np.array([spec[k] for k in keys], np.complex128)
+    return pyffo.pauli_propagate(gates, 1, x, z, c, **kw)
+
+
+def dense_heisenberg(gates, n, O):
+    """U_total^dag O U_total for a packed gate table (gate 0 acts first)."""
+    U = np.eye(2 ** n, dtype=complex)
+    for g in gates:
+        if g["kind"] == _abi.ROT:
+            letters = D.letters_from_words(int(g["x"][0]), int(g["z"][0]), n)
+            G = D.rot_unitary(letters, float(g["theta"]))
+        else:
+            G = D.CLIFFORD[int(g["kind"])](n, int(g["q0"]), int(g["q1"]))
+        U = G @ U
+    return U.conj().T @ O @ U
+
+
+def rot(letters, theta):
+    x, z = D.words_from_letters(letters)
+    g = np.zeros(1, _abi.PAULI_GATE_DTYPE)
+    g["kind"] = _abi.ROT
+    g["theta"] = theta
+    g["x"][0, 0] = x
+    g["z"][0, 0] = z
+    return g
+
+
+def test_single_rotation_closed_form_all_two_qubit_pairs():
+    """U^dag Q U = cos(t) Q + i sin(t) P Q for anticommuting P, Q, else Q."""
+    rng = np.random.default_rng(0)
+    n = 2
+    for P in itertools.product("IXYZ", repeat=n):
+        for Q in itertools.product("IXYZ", repeat=n):
+            P, Q = "".join(P), "".join(Q)
+            t = float(rng.uniform(-3, 3))
+            res = as_dict(run1(rot(P, t), {D.words_from_letters(Q): 1.0}))
+            want = D.pauli_decompose(D.rot_unitary(P, t).conj().T @ D.pauli_matrix(Q) @ D.rot_unitary(P, t), n)
+            for k in set(res) | set(want):
+                assert abs(res.get(k, 0) - want.get(k, 0)) < 1e-14, (P, Q, k)
+
+
+def random_circuit(rng, n, ng, p_cliff=0.25):
+    recs = []
+    for _ in range(ng):
+        if rng.random() < p_cliff:
+            kind = int(rng.integers(1, 6))
+            a, b = (int(v) for v in rng.choice(n, 2, replace=False))
+            recs.append(_abi.clifford_record(kind, a, b))
+        else:
+            letters = "".join(rng.choice(list("IXYZ"), n))
+            if set(letters) == {"I"}:
+                letters = "X" + letters[1:]
+            recs.append(rot(letters, float(rng.uniform(-math.pi, math.pi))))
+    return np.concatenate(recs)
+
+
+@pytest.mark.parametrize("seed", range(8))
+@pytest.mark.parametrize("batched", [True, False])
+def test_random_circuits_vs_dense(seed, batched):
+    """Untruncated propagation = dense U^dag O U, coefficient by coefficient."""
+    rng = np.random.default_rng(seed)
+    n = 3 if seed < 4 else 4
+    g = random_circuit(rng, n, 14)
+    spec = {}
+    for _ in range(3):
+        s = "".join(rng.choice(list("IXYZ"), n))
+        spec[D.words_from_letters(s)] = complex(rng.normal(), rng.normal())
+    O = sum(c * D.pauli_matrix(D.letters_from_words(k[0], k[1], n)) for k, c in spec.items())
+    want = D.pauli_decompose(dense_heisenberg(g, n, O), n)
+    res = as_dict(run1(g, spec, batched=batched))
+    for k in set(res) | set(want):
+        assert abs(res.get(k, 0) - want.get(k, 0)) < 1e-12, k
+    # Frobenius norm is conserved without truncation
+    assert abs(sum(abs(v) ** 2 for v in res.values()) - sum(abs(v) ** 2 for v in spec.values())) < 1e-12
+
+
+@pytest.mark.parametrize("seed", range(4))
+def test_product_state_expectation_vs_dense(seed):
+    rng = np.random.default_rng(50 + seed)
+    n = 5
+    g = random_circuit(rng, n, 20)
+    spec = {D.words_from_letters("Z" + "I" * (n - 1)): 1.0, D.words_from_letters("IXXII"): 0.5}
+    O = sum(c * D.pauli_matrix(D.letters_from_words(k[0], k[1], n)) for k, c in spec.items())
+    res = run1(g, spec)
+    for up in ([], [0], [1, 3], [0, 2, 4]):
+        b = sum(1 << q for q in up)
+        U = np.eye(2 ** n, dtype=complex)
+        v = D.basis_vector(n, up)
+        want = v.conj() @ dense_heisenberg(g, n, O) @ v
+        assert abs(pyffo.pauli_expect(res, [b]) - want) < 1e-12
+        assert abs(pyffo.pauli_expect_native(res, [b]) - want) < 1e-12
+
+
+def test_degree_filter_applies_to_partners_only():
+    """A kept term of degree > maxdegree survives; its partner is dropped."""
+    P = "XIII"
+    Q = "ZZZI"  # degree 3, anticommutes with P; partner Y Z Z I has degree 3
+    res = as_dict(run1(rot(P, 0.3), {D.words_from_letters(Q): 1.0}, maxdegree=2))
+    assert list(res) == [D.words_from_letters(Q)]
+    assert abs(res[D.words_from_letters(Q)] - math.cos(0.3)) < 1e-15
+    res = as_dict(run1(rot(P, 0.3), {D.words_from_letters(Q): 1.0}, maxdegree=3))
+    assert len(res) == 2
+
+
+def test_threshold_erases_equality_and_keeps_exact_zeros():
+    # |c| == mincoeff is erased (truncate.h:16-18)
+    c, s = math.cos(0.3), math.sin(0.3)
+    res = as_dict(run1(rot("X", 0.3), {D.words_from_letters("Z"): 1.0}, mincoeff=abs(s)))
+    assert list(res) == [D.words_from_letters("Z")]
+    # theta = 0 creates exact-zero partners; with mincoeff = 0 they are kept
+    res = as_dict(run1(rot("X", 0.0), {D.words_from_letters("Z"): 1.0}))
+    assert len(res) == 2 and res[D.words_from_letters("Y")] == 0
+    res = as_dict(run1(rot("X", 0.0), {D.words_from_letters("Z"): 1.0}, mincoeff=1e-300))
+    assert len(res) == 1
+
+
+def test_batched_vs_unbatched_schedules():
+    """Without truncation the schedule does not matter; with a threshold the
+    expectation agrees to the truncation scale (test/batched_test.py:10-56)."""
+    n = 6
+    tf = [("ZZ", (i, (i + 1) % n), -0.2) for i in range(n)] + [("X", (i,), -0.2) for i in range(n)]
+    g = np.concatenate([wl._pack(tf)] * 5)
+    spec = {D.words_from_letters("Z" + "I" * (n - 1)): 1.0}
+    a = as_dict(run1(g, spec, batched=True))
+    b = as_dict(run1(g, spec, batched=False))
+    assert set(a) == set(b) and all(abs(a[k] - b[k]) < 1e-12 for k in a)
+    ra, rb = run1(g, spec, batched=True, mincoeff=1e-3), run1(g, spec, batched=False, mincoeff=1e-3)
+    assert abs(pyffo.pauli_expect(ra, [0]) - pyffo.pauli_expect(rb, [0])) < 1e-3
+
+
+def test_xweight_truncation_period():
+    n = 4
+    tf = [("ZZ", (i, (i + 1) % n), 0.3) for i in range(n)] + [("X", (i,), 0.4) for i in range(n)]
+    g = np.concatenate([wl._pack(tf)] * 3)
+    spec = {D.words_from_letters("ZIII"): 1.0}
+    res = run1(g, spec, batched=False, max_xweight=1, xtrunc_period=len(g))
+    # the cap applies once, after the last rotation
+    assert all(bin(int(v)).count("1") <= 1 for v in res.x[:, 0])
+    free = run1(g, spec, batched=False)
+    assert len(free) > len(res)
+
+
+def test_topk_keeps_largest():
+    n = 4
+    tf = [("ZZ", (i, (i + 1) % n), 0.3) for i in range(n)] + [("X", (i,), 0.4) for i in range(n)]
+    g = np.concatenate([wl._pack(tf)] * 2)
+    spec = {D.words_from_letters("ZIII"): 1.0}
+    full = run1(g, spec, batched=True)
+    res = run1(g, spec, batched=False, topk=10)
+    assert len(res) <= 10
+
+
+def tfim(n, dt):
+    return wl._pack([("ZZ", (i, (i + 1) % n), -2 * dt) for i in range(n)] + [("X", (i,), -2 * dt) for i in range(n)])
+
+
+def heisenberg(n, dt):
+    g = []
+    for i in range(n):
+        j = (i + 1) % n
+        g += [("XX", (i, j), 2 * dt), ("YY", (i, j), 2 * dt), ("ZZ", (i, j), 2 * dt)]
+    return wl._pack(g)
+
+
+@pytest.mark.parametrize("strategy,threads", [(pyffo.SHARDED, 2), (pyffo.SHARDED, 4), (pyffo.SHARDED, 7),
+                                              (pyffo.OMP, 4), (pyffo.SHARDED, 32)])
+@pytest.mark.parametrize("case", ["tfim", "thresh", "maxdeg", "heis", "mixed"])
+def test_parallel_strategies_equal_serial(strategy, threads, case):
+    """The reference's omp/sharded consistency suites (test/omp_test.py,
+    test/sharded_test.py): same terms, coefficients to 1e-12."""
+    kw = {}
+    if case == "tfim":
+        g, spec = np.concatenate([tfim(6, 0.1)] * 5), {D.words_from_letters("ZIIIII"): 1.0}
+    elif case == "thresh":
+        g, spec, kw = np.concatenate([tfim(6, 0.1)] * 5), {D.words_from_letters("ZIIIII"): 1.0}, {"mincoeff": 1e-3}
+    elif case == "maxdeg":
+        g, spec, kw = np.concatenate([tfim(6, 0.1)] * 5), {D.words_from_letters("ZIIIII"): 1.0}, {"maxdegree": 3}
+    elif case == "heis":
+        g, spec = np.concatenate([heisenberg(4, 0.03)] * 5), {D.words_from_letters("ZIII"): 1.0}
+    else:
+        c = np.concatenate([_abi.clifford_record(_abi.H, 0), rot("ZI", 0.5), _abi.clifford_record(_abi.CNOT, 0, 1),
+                            rot("XX", 0.3)])
+        g, spec = np.concatenate([c] * 5), {D.words_from_letters("ZI"): 1.0}
+    a = run1(g, spec, **kw)
+    b = run1(g, spec, strategy=strategy, n_threads=threads, **kw)
+    da, db = as_dict(a), as_dict(b)
+    assert set(da) == set(db)
+    assert all(abs(da[k] - db[k]) < 1e-12 for k in da)
+    assert abs(pyffo.pauli_expect(a, [0]) - pyffo.pauli_expect(b, [0])) < 1e-12
+
+
+def test_su4_golden():
+    """fastfermion's own example: <Z15> on |0...0> = 0.008664 +- 1e-6 for
+    examples/data/su4circuit-16.txt with maxdegree 4."""
+    g = wl.read_circuit(os.path.join(GOLDEN, "su4circuit-16.txt"))
+    assert len(g) == 1440
+    x, z, c = wl.single_pauli("Z", [15], 16)
+    r = pyffo.pauli_propagate(g, 1, x, z, c, maxdegree=4, strategy=pyffo.SHARDED, n_threads=4)
+    assert abs(pyffo.pauli_expect(r, [0]) - 0.008664) <= 1e-6
+
+
+# ---------------------------------------------------------------------------
+# Majorana
+# ---------------------------------------------------------------------------
+
+def mrot(S, theta):
+    r = (len(S) >> 1) & 1
+    return wl.pack_majorana([(S, theta, r)])
+
+
+def mrun(gates, spec, **kw):
+    keys = list(spec)
+    m = np.array([[k] for k in keys], np.uint64).reshape(-1, 1)
+    c = np.array([spec[k] for k in keys], np.complex128)
+    res = pyffo.majorana_propagate(gates, 1, m, c, **kw)
+    return {int(res.x[i, 0]): res.c[i] for i in range(len(res.c))}, res
+
+
+def mrot_unitary(S, theta, modes):
+    """exp(-i theta/2 i^r m_S), r = 1 iff |S| mod 4 in {2, 3}."""
+    r = (len(S) >> 1) & 1
+    M = (1j) ** r * D.majorana_matrix(S, modes)
+    return D.expm(-0.5j * theta * M)
+
+
+def test_majorana_single_rotation_all_pairs():
+    modes = 2
+    nm = 2 * modes
+    rng = np.random.default_rng(3)
+    subsets = [S for r in range(nm + 1) for S in itertools.combinations(range(nm), r)]
+    for S in subsets:
+        if not S:
+            continue
+        for T in subsets:
+            t = float(rng.uniform(-2, 2))
+            res, _ = mrun(mrot(list(S), t), {sum(1 << a for a in T): 1.0})
+            U = mrot_unitary(S, t, modes)
+            want = D.majorana_decompose(U.conj().T @ D.majorana_matrix(T, modes) @ U, modes)
+            for k in set(res) | set(want):
+                assert abs(res.get(k, 0) - want.get(k, 0)) < 1e-13, (S, T, k)
+
+
+@pytest.mark.parametrize("seed", range(5))
+def test_majorana_random_circuits_vs_dense(seed):
+    rng = np.random.default_rng(20 + seed)
+    modes = 3
+    nm = 2 * modes
+    gates = []
+    for _ in range(10):
+        d = int(rng.integers(1, 5))
+        S = sorted(rng.choice(nm, d, replace=False).tolist())
+        gates.append((S, float(rng.uniform(-2, 2)), (d >> 1) & 1))
+    g = wl.pack_majorana(gates)
+    spec = {0b11: 0.5j, 0b100: -0.3, 0b110110: 0.7}
+    O = sum(c * D.majorana_matrix([a for a in range(nm) if (k >> a) & 1], modes) for k, c in spec.items())
+    U = np.eye(2 ** modes, dtype=complex)
+    for S, t, r in gates:
+        U = mrot_unitary(S, t, modes) @ U
+    want = D.majorana_decompose(U.conj().T @ O @ U, modes)
+    res, raw = mrun(g, spec)
+    for k in set(res) | set(want):
+        assert abs(res.get(k, 0) - want.get(k, 0)) < 1e-12, k
+    for occ in (0, 0b101, 0b11):
+        # JW: mode j occupied <-> qubit j in |1>
+        v = D.basis_vector(modes, [j for j in range(modes) if (occ >> j) & 1])
+        assert abs(pyffo.majorana_expect(raw, [occ]) - v.conj() @ (U.conj().T @ O @ U) @ v) < 1e-12
+
+
+def test_majorana_truncation_semantics():
+    # degree filter on partners only, threshold after every gate
+    # term m1 m2 m3 (degree 3 > 2) anticommutes with m0 m1; its partner m0 m2 m3 is dropped
+    res, _ = mrun(mrot([0, 1], 0.4), {0b1110: 1.0}, maxdegree=2)
+    assert list(res) == [0b1110]
+    res, _ = mrun(mrot([0, 1], 0.4), {0b1110: 1.0}, maxdegree=3)
+    assert len(res) == 2
+    # |c| == mincoeff is erased after the gate
+    res, _ = mrun(mrot([0, 1], 0.4), {0b1110: 1.0}, mincoeff=abs(math.sin(0.4)))
+    assert list(res) == [0b1110]
