@@ -45,7 +45,8 @@ class Profile(ctypes.Structure):
                 ("seconds_upload", ctypes.c_double), ("seconds_propagate", ctypes.c_double),
                 ("seconds_download", ctypes.c_double), ("outs", ctypes.c_int64),
                 ("compactions", ctypes.c_int64), ("run_launches", ctypes.c_int64),
-                ("seconds_kernel", ctypes.c_double)]
+                ("seconds_kernel", ctypes.c_double), ("timed_launches", ctypes.c_int64),
+                ("seconds_enqueue", ctypes.c_double)]
 
 
 _lib = None
@@ -204,9 +205,10 @@ class Context:
         self._check(self._L.ffx_expect_last(self._h, _p(st), _p(ex)))
         return complex(ex[0], ex[1])
 
-    def set_kernel_timing(self, on: bool) -> None:
-        """Bracket every propagation kernel launch with HIP events (see profile())."""
-        self._check(self._L.ffx_set_kernel_timing(self._h, int(bool(on))))
+    def set_kernel_timing(self, every: int) -> None:
+        """Bracket every `every`-th propagation kernel launch with HIP events
+        (0: off); profile() then reports seconds_kernel over timed_launches."""
+        self._check(self._L.ffx_set_kernel_timing(self._h, int(every)))
 
     def profile(self) -> dict:
         p = Profile()

@@ -45,12 +45,20 @@ def hip_sources():
         [os.path.join(INCLUDE, "fastfermion_amd.h")]
 
 
-def build_lib(force=True):
+def build_lib(force=True, jobs=None):
+    """Compile every csrc/*.hip translation unit in parallel, then link libffamd.so."""
     out = os.path.join(HERE, "libffamd.so")
     if not force and _newer(out, hip_sources()):
         return out
+    units = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = [os.path.join(objdir, os.path.basename(u)[:-4] + ".o") for u in units]
+    jobs = jobs or min(len(units), max(1, min(16, os.cpu_count() or 1)))
+    with ThreadPoolExecutor(jobs) as ex:
+        list(ex.map(lambda uo: _run([HIPCC, *HIP_FLAGS, "-c", "-o", uo[1], uo[0]]), zip(units, objs)))
     tmp = out + ".tmp"
-    _run([HIPCC, *HIP_FLAGS, "-shared", "-o", tmp, os.path.join(CSRC, "ffx_lib.hip")])
+    _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs])
     os.replace(tmp, out)
     return out
 

@@ -29,6 +29,7 @@ struct Epochs {
     uint32_t* app;   // appends of the epoch
     int32_t* dd;     // erased-count change of the epoch
     uint32_t* flags; // [0] overflow raised, [1] epoch that raised it
+    int32_t* sd;     // windows: live-count change at each rotation of the epoch's window
 };
 
 __device__ __forceinline__ bool run_stopped(const Epochs& ep) { return atomic_load_u32(&ep.flags[0]) != 0; }
@@ -38,6 +39,22 @@ __device__ __forceinline__ void epoch_in(const Epochs& ep, uint32_t e, uint32_t&
     dead = (uint32_t)((int32_t)ep.dead[e - 1] + ep.dd[e - 1]);
 }
 
+// Entry of a step kernel: the stop flag and the epoch's input counts are read
+// in one round of independent loads.  Returns false if the run has stopped.
+__device__ __forceinline__ bool step_enter(const Epochs& ep, uint32_t e, uint32_t& n, uint32_t& dead) {
+    const uint32_t stop = atomic_load_u32(&ep.flags[0]);
+    const uint32_t n0 = ep.n[e - 1], a0 = ep.app[e - 1], d0 = ep.dead[e - 1];
+    const int32_t dd0 = ep.dd[e - 1];
+    if (stop) return false;
+    n = n0 + a0;
+    dead = (uint32_t)((int32_t)d0 + dd0);
+    if (blockIdx.x == 0 && threadIdx.x == 0) {
+        ep.n[e] = n;
+        ep.dead[e] = dead;
+    }
+    return true;
+}
+
 // One rotation.  Each block owns a contiguous chunk of slots.
 template <class Alg, class CP, int K>
 __global__ void __launch_bounds__(kBlockRot) k_rot_async(Store<CP, K> st, const typename CP::store_t* cin,
@@ -45,17 +62,13 @@ __global__ void __launch_bounds__(kBlockRot) k_rot_async(Store<CP, K> st, const 
                                                          RotParams<CP> rp, Epochs ep, uint32_t e) {
     constexpr int NP = Alg::NP;
     using val_t = typename CP::val_t;
-    if (run_stopped(ep)) return;
     uint32_t n, dead;
-    epoch_in(ep, e, n, dead);
-    if (blockIdx.x == 0 && threadIdx.x == 0) {
-        ep.n[e] = n;
-        ep.dead[e] = dead;
-    }
+    if (!step_enter(ep, e, n, dead)) return;
     const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
     const uint32_t chunk = (per + blockDim.x - 1) / blockDim.x * blockDim.x;
     const uint32_t begin = blockIdx.x * chunk;
     const uint32_t end = min(n, begin + chunk);
+    if (begin >= end) return;  // block-uniform: nothing to do, nothing to report
     __shared__ uint32_t sbase;
     int dd = 0;
     for (uint32_t b0 = begin; b0 < end; b0 += blockDim.x) {
@@ -108,7 +121,7 @@ __global__ void __launch_bounds__(kBlock) k_cmp_count(Store<CP, K> st, Epochs ep
     }
 }
 
-__global__ void __launch_bounds__(kBlock) k_cmp_scan(Epochs ep, uint32_t e_prev, uint32_t* counts) {
+static __global__ void __launch_bounds__(kBlock) k_cmp_scan(Epochs ep, uint32_t e_prev, uint32_t* counts) {
     if (run_stopped(ep)) return;
     const uint32_t n = ep.n[e_prev] + ep.app[e_prev];
     const uint32_t dead = (uint32_t)((int32_t)ep.dead[e_prev] + ep.dd[e_prev]);
@@ -279,7 +292,7 @@ __global__ void __launch_bounds__(kBlock) k_cliff_async(Store<CP, K> st, const C
 }
 
 // Remove index entries of slots >= n (the appends of a step that overflowed).
-__global__ void k_index_rollback(uint64_t* index, size_t nslots, uint32_t n) {
+static __global__ void k_index_rollback(uint64_t* index, size_t nslots, uint32_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nslots; i += (size_t)gridDim.x * blockDim.x) {
         const uint64_t e = index[i];
         if (e != kEmpty && (uint32_t)e >= n) index[i] = kEmpty;

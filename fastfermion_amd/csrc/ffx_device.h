@@ -40,22 +40,39 @@ struct Ctrl {
     unsigned long long n_compact;  // compactions run
 };
 
-__device__ __forceinline__ uint64_t mix64(uint64_t h) {
-    // splitmix64 finalizer (Steele, Lea, Flood 2014)
-    h ^= h >> 30;
-    h *= 0xbf58476d1ce4e5b9ull;
-    h ^= h >> 27;
-    h *= 0x94d049bb133111ebull;
-    h ^= h >> 31;
+// Index hash = hmix(L(key)).  L is GF(2)-linear, L(a ^ b) = L(a) ^ L(b): word p
+// contributes rotl(w, r1) ^ rotl(w, r2) ^ rotl(w, r3), an invertible map (odd
+// number of rotations), with rotation triples that are not translates of one
+// another, so keys differing in one word, or in one bit of each of two words,
+// never share L.  Linearity lets a window derive the hash of every orbit
+// string from the slot's own L and a per-window table (no per-string key
+// hashing).  hmix is a bijective xorshift-multiply finalizer.
+constexpr int kHashRot[8][3] = {{10, 21, 61}, {4, 26, 42}, {5, 35, 53}, {7, 24, 38},
+                                {4, 33, 59}, {6, 16, 36}, {4, 28, 53}, {8, 37, 61}};
+
+__host__ __device__ __forceinline__ uint64_t rotl64(uint64_t x, int r) { return (x << r) | (x >> (64 - r)); }
+
+template <int NP>
+__host__ __device__ __forceinline__ uint64_t lin_hash(const uint64_t* k) {
+    uint64_t h = 0;
+#pragma unroll
+    for (int p = 0; p < NP; ++p)
+        h ^= rotl64(k[p], kHashRot[p][0]) ^ rotl64(k[p], kHashRot[p][1]) ^ rotl64(k[p], kHashRot[p][2]);
+    return h;
+}
+
+__host__ __device__ __forceinline__ uint64_t hmix(uint64_t h) {
+    h ^= h >> 32;
+    h *= 0xD6E8FEB86659FD93ull;
+    h ^= h >> 32;
+    h *= 0xD6E8FEB86659FD93ull;
+    h ^= h >> 32;
     return h;
 }
 
 template <int NP>
 __device__ __forceinline__ uint64_t key_hash(const uint64_t (&k)[NP]) {
-    uint64_t h = 0x9e3779b97f4a7c15ull;
-#pragma unroll
-    for (int p = 0; p < NP; ++p) h = mix64(h ^ k[p]) + (uint64_t)p;
-    return mix64(h);
+    return hmix(lin_hash<NP>(k));
 }
 
 __device__ __forceinline__ int popc(uint64_t v) { return __popcll(v); }
@@ -207,6 +224,24 @@ struct PauliAlg {
         for (int w = 0; w < W; ++w) j += popc(q[w] & q[W + w]) + 2 * popc(g.p[W + w] & q[w]) - popc(r[w] & r[W + w]);
         return ((j & 3) == 3) ? 1 : -1;
     }
+    // sign(q, r, g) factored for the window replay: it depends on q and r only
+    // through quad(q), quad(r) (|x & z| mod 4) and lin(q, g) (|P.z & q.x| mod 2,
+    // linear in q).
+    __device__ static int quad(const uint64_t (&k)[NP]) {
+        int s = 0;
+#pragma unroll
+        for (int w = 0; w < W; ++w) s += popc(k[w] & k[W + w]);
+        return s & 3;
+    }
+    __host__ __device__ static int lin(const uint64_t* k, const Gate& g) {
+        int s = 0;
+        for (int w = 0; w < W; ++w) s += __builtin_popcountll(g.p[W + w] & k[w]);
+        return s & 1;
+    }
+    __device__ static int sign_parts(const Gate& g, int quad_q, int quad_r, int lin_q) {
+        const int j = g.pxz + quad_q + 2 * lin_q - quad_r;
+        return ((j & 3) == 3) ? 1 : -1;
+    }
 };
 
 // Majorana strings with W occupation words (src/majorana_algebra.h:30-36).
@@ -253,6 +288,19 @@ struct MajoranaAlg {
         const int e = rq + 1 + g.r - rr;
         int s = ((e & 3) == 0) ? 1 : -1;
         return (mp & 1) ? -s : s;
+    }
+    // sign(q, r, g) factored for the window replay: quad = r(string) =
+    // (degree >> 1) & 1, lin(q, g) = merge parity |q & sm| mod 2 (linear in q).
+    __device__ static int quad(const uint64_t (&k)[NP]) { return (degree(k) >> 1) & 1; }
+    __host__ __device__ static int lin(const uint64_t* k, const Gate& g) {
+        int s = 0;
+        for (int w = 0; w < W; ++w) s += __builtin_popcountll(k[w] & g.sm[w]);
+        return s & 1;
+    }
+    __device__ static int sign_parts(const Gate& g, int quad_q, int quad_r, int lin_q) {
+        const int e = quad_q + 1 + g.r - quad_r;
+        const int s = ((e & 3) == 0) ? 1 : -1;
+        return lin_q ? -s : s;
     }
 };
 

@@ -79,6 +79,29 @@ __device__ __forceinline__ int block_sum_int(int v) {
     return t;
 }
 
+// Block-wide sums of N ints at once (one barrier pair), results valid in thread 0.
+template <int N>
+__device__ __forceinline__ void block_sum_ints(int (&v)[N]) {
+    __shared__ int red[kMaxWaves][N];
+#pragma unroll
+    for (int q = 0; q < N; ++q)
+        for (int o = 32; o > 0; o >>= 1) v[q] += __shfl_down(v[q], o, 64);
+    const int wid = threadIdx.x >> 6;
+    if (__lane_id() == 0)
+#pragma unroll
+        for (int q = 0; q < N; ++q) red[wid][q] = v[q];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+#pragma unroll
+        for (int q = 0; q < N; ++q) {
+            int t = 0;
+            for (int w = 0; w < (int)(blockDim.x >> 6); ++w) t += red[w][q];
+            v[q] = t;
+        }
+    }
+    __syncthreads();
+}
+
 // Block-wide exclusive scan of a uint32; returns the prefix, total in *total.
 __device__ __forceinline__ uint32_t block_exclusive_scan(uint32_t v, uint32_t* total) {
     __shared__ uint32_t wsum[kMaxWaves];
@@ -222,6 +245,19 @@ __device__ __forceinline__ void put_slot(const Store<CP, K>& st, uint64_t* key, 
     index_insert(st.index, st.hmask, key_hash<NP>(r), (uint32_t)slot, &st.ctrl->index_full);
 }
 
+// Same with the key's index hash supplied by the caller.
+template <int NP, class CP, int K>
+__device__ __forceinline__ void put_slot_h(const Store<CP, K>& st, uint64_t* key, typename CP::store_t* cout,
+                                           uint64_t slot, const uint64_t (&r)[NP],
+                                           const typename CP::val_t (&vapp)[K], uint64_t h) {
+    const size_t cap = st.cap;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) key[q * cap + slot] = r[q];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) CP::store(cout + (size_t)kk * CP::NS * cap, cap, slot, vapp[kk]);
+    index_insert(st.index, st.hmask, h, (uint32_t)slot, &st.ctrl->index_full);
+}
+
 // Grid-stride sweep of one rotation with wave-aggregated appends.
 template <class Alg, class CP, int K>
 __device__ __forceinline__ int rot_sweep(const Store<CP, K>& st, const uint64_t* key,
@@ -352,7 +388,7 @@ __global__ void __launch_bounds__(kBlock) k_clifford(Store<CP, K> st, const Clif
 // ---------------------------------------------------------------------------
 // Index maintenance
 // ---------------------------------------------------------------------------
-__global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
+static __global__ void k_fill_u64(uint64_t* p, size_t n, uint64_t v) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x)
         p[i] = v;
 }
@@ -399,7 +435,7 @@ __global__ void __launch_bounds__(kBlock) k_count_live(Store<CP, K> st, uint32_t
 }
 
 // Single-block exclusive scan of counts[0..nb) in place; total to *total.
-__global__ void __launch_bounds__(kBlock) k_scan_counts(uint32_t* counts, uint32_t nb, uint32_t* total) {
+static __global__ void __launch_bounds__(kBlock) k_scan_counts(uint32_t* counts, uint32_t nb, uint32_t* total) {
     uint32_t carry = 0;
     for (uint32_t b0 = 0; b0 < nb; b0 += kBlock) {
         const uint32_t i = b0 + threadIdx.x;
@@ -590,7 +626,7 @@ __global__ void __launch_bounds__(kBlock) k_expect_partial(Store<CP, K> st, Stat
     }
 }
 
-__global__ void k_expect_final(const double* partial, int nb, double* out) {
+static __global__ void k_expect_final(const double* partial, int nb, double* out) {
     if (threadIdx.x == 0 && blockIdx.x == 0) {
         double a = 0.0, b = 0.0;
         for (int i = 0; i < nb; ++i) {

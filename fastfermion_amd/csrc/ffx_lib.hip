@@ -1,941 +1,16 @@
-// ffx_lib.hip -- host engine and C ABI of libffamd.so (declared in
-// include/fastfermion_amd.h).
-//
-// The host side restates the control flow of the reference drivers --
-// reverse walk of the circuit, Clifford blocks, greedy commuting batches,
-// truncation points (src/pauli/propagate.h:297-366) and the per-gate threshold of
-// the Majorana driver (src/majorana_propagation.h:18-33) -- as a list of device
-// steps, then runs every step as HIP kernels on one stream.
-#include <hip/hip_runtime.h>
+// ffx_lib.hip -- C ABI of libffamd.so (declared in include/fastfermion_amd.h):
+// argument checks, then dispatch on the key width to the drivers instantiated
+// in ffx_w1.hip .. ffx_w4.hip.
+#include "ffx_engine.h"
 
-#include <algorithm>
-#include <chrono>
-#include <cmath>
-#include <cstdio>
-#include <cstring>
-#include <string>
-#include <type_traits>
-#include <utility>
-#include <vector>
+using namespace ffx_host;
 
-#include "../../include/fastfermion_amd.h"
-#include "ffx_async.h"
-
-using namespace ffx;
-
-struct ffx_ctx {
-    int device = 0;
-    hipStream_t stream = nullptr;
-    std::string err;
-    // term store
-    uint64_t* key = nullptr;
-    uint64_t* key2 = nullptr;
-    void* coef = nullptr;
-    void* coef2 = nullptr;
-    uint64_t* index = nullptr;
-    size_t cap = 0;      // slots
-    size_t hslots = 0;   // index entries (power of two, >= 2*cap)
-    size_t key_planes = 0, coef_planes = 0, coef_elem = 0;  // layout of the current store
-    Ctrl* ctrl = nullptr;
-    Ctrl* ctrl_host = nullptr;  // pinned
-    uint32_t* counts = nullptr;
-    size_t counts_cap = 0;
-    uint32_t* total = nullptr;
-    double* partial = nullptr;
-    double* result = nullptr;
-    double* result_host = nullptr;  // pinned, 2 doubles
-    void* staging = nullptr;
-    size_t staging_bytes = 0;
-    CliffGate* cliff = nullptr;
-    size_t cliff_cap = 0;
-    uint32_t* epochs = nullptr;
-    size_t epochs_cap = 0;
-    int n_cu = 0;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
-    int kernel_timing = 0;
-    std::vector<hipEvent_t> kev;  // event pairs around rotation launches
-    // description of the device-resident result
-    int last_kind = 0;  // 0 none, 1 Pauli, 2 Majorana
-    int last_W = 0, last_K = 0, last_prec = 0;
-    ffx_profile prof{};
-    int64_t launches = 0;
-};
-
-namespace {
-
-#define HIPCHK(expr)                                                                                   \
-    do {                                                                                               \
-        hipError_t _e = (expr);                                                                        \
-        if (_e != hipSuccess) {                                                                        \
-            ctx->err = std::string("HIP error: ") + hipGetErrorString(_e) + " at " #expr;              \
-            return _e == hipErrorOutOfMemory ? FFX_ENOMEM : FFX_EHIP;                                  \
-        }                                                                                              \
-    } while (0)
-
-#define CHK(expr)                   \
-    do {                            \
-        int _r = (expr);            \
-        if (_r != FFX_OK) return _r; \
-    } while (0)
-
-inline double now_s() {
-    return std::chrono::duration<double>(std::chrono::steady_clock::now().time_since_epoch()).count();
-}
-
-int set_err(ffx_ctx* ctx, int code, const std::string& msg) {
-    ctx->err = msg;
-    return code;
-}
-
-size_t pow2ceil(size_t v) {
-    size_t p = 1;
-    while (p < v) p <<= 1;
-    return p;
-}
-
-template <class T>
-int dev_alloc(ffx_ctx* ctx, T** p, size_t bytes) {
-    *p = nullptr;
-    if (bytes == 0) bytes = 256;
-    hipError_t e = hipMalloc((void**)p, bytes);
-    if (e != hipSuccess) {
-        *p = nullptr;
-        (void)hipGetLastError();
-        return set_err(ctx, FFX_ENOMEM, "device allocation of " + std::to_string(bytes) + " bytes failed");
-    }
-    return FFX_OK;
-}
-
-template <class T>
-void dev_free(T*& p) {
-    if (p) (void)hipFree((void*)p);
-    p = nullptr;
-}
-
-void free_store(ffx_ctx* ctx) {
-    dev_free(ctx->key);
-    dev_free(ctx->key2);
-    dev_free(ctx->coef);
-    dev_free(ctx->coef2);
-    dev_free(ctx->index);
-    ctx->cap = ctx->hslots = 0;
-}
-
-unsigned grid_for(size_t n, size_t per_block, unsigned max_grid = 65535) {
-    size_t g = (n + per_block - 1) / per_block;
-    if (g < 1) g = 1;
-    if (g > max_grid) g = max_grid;
-    return (unsigned)g;
-}
-
-// One device step of a propagation.
-struct Step {
-    int kind = 0;        // 0 rotation, 1 Clifford block
-    int64_t gate = 0;    // rotation: gate index
-    int trunc = 0;       // rotation: truncate_all runs after it
-    int rc = 0;          // rotation: rot_count argument of that truncate_all
-    int64_t cb = 0, ce = 0;  // Clifford block [cb, ce)
-};
-
-template <int W>
-bool rot_commute(const ffx_pauli_gate& a, const ffx_pauli_gate& b) {
-    int p1 = 0, p2 = 0;
-    for (int w = 0; w < W; ++w) {
-        p1 += __builtin_popcountll(a.x[w] & b.z[w]);
-        p2 += __builtin_popcountll(a.z[w] & b.x[w]);
-    }
-    return (p1 & 1) == (p2 & 1);
-}
-
-// Schedule of src/pauli/propagate.h:318-366 (serial driver): the circuit is
-// walked from the last gate to the first; runs of Clifford gates form blocks;
-// rotations either truncate after every gate (batched = false) or are grouped by
-// batch_commuting_gates (propagate.h:272-291) and truncate once per batch.
-template <int W>
-std::vector<Step> pauli_schedule(const ffx_pauli_gate* g, int64_t n, bool batched) {
-    std::vector<Step> steps;
-    int rot_count = 0;
-    bool pending = false;
-    int64_t cend = 0;
-    std::vector<int64_t> buf;
-    auto flush = [&]() {
-        if (buf.empty()) return;
-        size_t s = 0;
-        while (s < buf.size()) {
-            size_t e = s + 1;
-            while (e < buf.size()) {
-                bool ok = true;
-                for (size_t t = s; t < e; ++t)
-                    if (!rot_commute<W>(g[buf[e]], g[buf[t]])) {
-                        ok = false;
-                        break;
-                    }
-                if (!ok) break;
-                ++e;
-            }
-            for (size_t t = s; t < e; ++t) {
-                Step st;
-                st.kind = 0;
-                st.gate = buf[t];
-                steps.push_back(st);
-            }
-            rot_count += (int)(e - s);
-            steps.back().trunc = 1;
-            steps.back().rc = rot_count - 1;
-            s = e;
-        }
-        buf.clear();
-    };
-    for (int64_t i = n - 1; i >= 0; --i) {
-        if (g[i].kind != FFX_ROT) {
-            if (batched) flush();
-            if (!pending) {
-                cend = i;
-                pending = true;
-            }
-        } else {
-            if (pending) {
-                Step c;
-                c.kind = 1;
-                c.cb = i + 1;
-                c.ce = cend + 1;
-                steps.push_back(c);
-                pending = false;
-            }
-            if (batched)
-                buf.push_back(i);
-            else {
-                Step st;
-                st.kind = 0;
-                st.gate = i;
-                st.trunc = 1;
-                st.rc = rot_count;
-                steps.push_back(st);
-                rot_count++;
-            }
-        }
-    }
-    if (batched) flush();
-    if (pending) {
-        Step c;
-        c.kind = 1;
-        c.cb = 0;
-        c.ce = cend + 1;
-        steps.push_back(c);
-    }
-    return steps;
-}
-
-// Gate factors: one glibc sincos() call per angle, which is what GCC -O2 makes of
-// the reference's cos(theta); sin(theta) pair (src/pauli/propagate.h:76-77,
-// src/majorana_gates.h:86-88).  sincos and a separate sin can differ by one ulp.
-inline void gate_trig(double theta, double& c, double& s) { ::sincos(theta, &s, &c); }
-
-template <class CP>
-typename CP::fac_t make_fac(double v);
-template <>
-F64::fac_t make_fac<F64>(double v) {
-    return v;
-}
-template <>
-F32::fac_t make_fac<F32>(double v) {
-    return (float)v;
-}
-template <>
-F32C::fac_t make_fac<F32C>(double v) {
-    float h = (float)v;
-    return F32C::fac_t{h, (float)(v - (double)h)};
-}
-
-// Engine over one term store; Alg = PauliAlg<W> or MajoranaAlg<W>.
-template <class Alg, class CP, int K>
-struct Engine {
-    static constexpr int NP = Alg::NP;
-    using store_t = typename CP::store_t;
-    ffx_ctx* ctx;
-    uint32_t n_host = 0;     // slots in use (exact after sync)
-    uint32_t dead_host = 0;  // erased slots (exact after sync)
-
-    explicit Engine(ffx_ctx* c) : ctx(c) {}
-
-    Store<CP, K> store() const {
-        Store<CP, K> s;
-        s.key = ctx->key;
-        s.coef = (store_t*)ctx->coef;
-        s.coef2 = (store_t*)ctx->coef2;
-        s.index = ctx->index;
-        s.hmask = ctx->hslots - 1;
-        s.cap = ctx->cap;
-        s.ctrl = ctx->ctrl;
-        return s;
-    }
-
-    size_t coef_planes() const { return (size_t)K * CP::NS; }
-
-    int launch_check() {
-        ctx->launches++;
-        hipError_t e = hipGetLastError();
-        if (e != hipSuccess) return set_err(ctx, FFX_EHIP, std::string("kernel launch: ") + hipGetErrorString(e));
-        return FFX_OK;
-    }
-
-    // Index over slots [0, n_host).  The control block's n / n_dead are set from
-    // the host values first: every later kernel of the step-wise paths reads them.
-    int rebuild_index() {
-        if (n_host > ctx->cap) return set_err(ctx, FFX_EHIP, "internal error: slot count exceeds capacity");
-        HIPCHK(hipMemcpyAsync(&ctx->ctrl->n, &n_host, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(&ctx->ctrl->n_dead, &dead_host, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL(k_fill_u64, dim3(grid_for(ctx->hslots, kBlock, 8192)), dim3(kBlock), 0, ctx->stream,
-                           ctx->index, ctx->hslots, kEmpty);
-        CHK(launch_check());
-        hipLaunchKernelGGL(k_index_build<NP>, dim3(grid_for(n_host, kBlock, 16384)), dim3(kBlock), 0, ctx->stream,
-                           ctx->key, ctx->cap, ctx->index, ctx->hslots - 1, ctx->ctrl, n_host);
-        return launch_check();
-    }
-
-    // Make room for `need` slots; existing slots [0, n_host) are preserved.
-    int reserve(size_t need) {
-        if (need <= ctx->cap && ctx->key_planes == NP && ctx->coef_planes == coef_planes() &&
-            ctx->coef_elem == sizeof(store_t))
-            return FFX_OK;
-        if (need > 0x7FFFFFF0ull) return set_err(ctx, FFX_ECAPACITY, "term count exceeds 2^31 slots");
-        size_t ncap = pow2ceil(std::max<size_t>(need, 1024));
-        if (ncap < 2 * ctx->cap && ctx->key_planes == NP) ncap = std::max<size_t>(ncap, 2 * ctx->cap);
-        uint64_t *nkey = nullptr, *nkey2 = nullptr, *nindex = nullptr;
-        void *ncoef = nullptr, *ncoef2 = nullptr;
-        const size_t cbytes = coef_planes() * ncap * sizeof(store_t);
-        int rc = dev_alloc(ctx, &nkey, NP * ncap * sizeof(uint64_t));
-        if (rc == FFX_OK) rc = dev_alloc(ctx, &nkey2, NP * ncap * sizeof(uint64_t));
-        if (rc == FFX_OK) rc = dev_alloc(ctx, &ncoef, cbytes);
-        if (rc == FFX_OK) rc = dev_alloc(ctx, &ncoef2, cbytes);
-        if (rc == FFX_OK) rc = dev_alloc(ctx, &nindex, 2 * ncap * sizeof(uint64_t));
-        if (rc != FFX_OK) {
-            dev_free(nkey);
-            dev_free(nkey2);
-            dev_free(ncoef);
-            dev_free(ncoef2);
-            dev_free(nindex);
-            return rc;
-        }
-        if (n_host > 0 && ctx->key && ctx->key_planes == NP && ctx->coef_planes == coef_planes()) {
-            for (int q = 0; q < NP; ++q)
-                HIPCHK(hipMemcpyAsync(nkey + q * ncap, ctx->key + q * ctx->cap, n_host * sizeof(uint64_t),
-                                      hipMemcpyDeviceToDevice, ctx->stream));
-            for (size_t q = 0; q < coef_planes(); ++q)
-                HIPCHK(hipMemcpyAsync((store_t*)ncoef + q * ncap, (store_t*)ctx->coef + q * ctx->cap,
-                                      n_host * sizeof(store_t), hipMemcpyDeviceToDevice, ctx->stream));
-        }
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        free_store(ctx);
-        ctx->key = nkey;
-        ctx->key2 = nkey2;
-        ctx->coef = ncoef;
-        ctx->coef2 = ncoef2;
-        ctx->index = nindex;
-        ctx->cap = ncap;
-        ctx->hslots = 2 * ncap;
-        ctx->key_planes = NP;
-        ctx->coef_planes = coef_planes();
-        ctx->coef_elem = sizeof(store_t);
-        return rebuild_index();
-    }
-
-    int ensure_counts(size_t nb) {
-        if (nb + 1 <= ctx->counts_cap) return FFX_OK;
-        dev_free(ctx->counts);
-        size_t c = pow2ceil(nb + 1);
-        CHK(dev_alloc(ctx, &ctx->counts, c * sizeof(uint32_t)));
-        ctx->counts_cap = c;
-        return FFX_OK;
-    }
-
-    int sync_ctrl() {
-        HIPCHK(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        n_host = ctx->ctrl_host->n;
-        dead_host = ctx->ctrl_host->n_dead;
-        if (ctx->ctrl_host->overflow) return set_err(ctx, FFX_EHIP, "internal error: term store overflow");
-        if (ctx->ctrl_host->index_full) return set_err(ctx, FFX_EHIP, "internal error: hash index full");
-        return FFX_OK;
-    }
-
-    // Drop erased slots (order preserving) and rebuild the index.
-    int compact() {
-        if (n_host == 0) return FFX_OK;
-        const size_t nb = (n_host + kTile - 1) / kTile;
-        CHK(ensure_counts(nb));
-        Store<CP, K> s = store();
-        hipLaunchKernelGGL((k_count_live<CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, s, ctx->counts);
-        CHK(launch_check());
-        hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->counts, (uint32_t)nb,
-                           ctx->total);
-        CHK(launch_check());
-        hipLaunchKernelGGL((k_compact<NP, CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, s, ctx->counts,
-                           ctx->key2, (store_t*)ctx->coef2);
-        CHK(launch_check());
-        uint32_t total = 0;
-        HIPCHK(hipMemcpyAsync(&total, ctx->total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        std::swap(ctx->key, ctx->key2);
-        std::swap(ctx->coef, ctx->coef2);
-        n_host = total;
-        dead_host = 0;
-        HIPCHK(hipMemcpyAsync(&ctx->ctrl->n, &n_host, sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipMemcpyAsync(&ctx->ctrl->n_dead, &dead_host, sizeof(uint32_t), hipMemcpyHostToDevice,
-                              ctx->stream));
-        return rebuild_index();
-    }
-
-    int upload(const uint64_t* hx, const uint64_t* hz, const double* re, const double* im, int64_t n0) {
-        n_host = 0;
-        dead_host = 0;
-        CHK(reserve(std::max<size_t>(2 * (size_t)n0 + 64, 1024)));
-        Ctrl c{};
-        c.n = (uint32_t)n0;
-        HIPCHK(hipMemcpyAsync(ctx->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
-        if (n0 > 0) {
-            constexpr int W = Alg::W;
-            const size_t kbytes = (size_t)n0 * W * sizeof(uint64_t);
-            const size_t need = (Alg::kPauli ? 2 : 1) * kbytes + 2 * (size_t)n0 * sizeof(double);
-            CHK(ensure_staging(need));
-            char* sp = (char*)ctx->staging;
-            uint64_t* dx = (uint64_t*)sp;
-            uint64_t* dz = Alg::kPauli ? (uint64_t*)(sp + kbytes) : nullptr;
-            double* dre = (double*)(sp + (Alg::kPauli ? 2 : 1) * kbytes);
-            double* dim = dre + n0;
-            HIPCHK(hipMemcpyAsync(dx, hx, kbytes, hipMemcpyHostToDevice, ctx->stream));
-            if (Alg::kPauli) HIPCHK(hipMemcpyAsync(dz, hz, kbytes, hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(dre, re, n0 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-            if (im)
-                HIPCHK(hipMemcpyAsync(dim, im, n0 * sizeof(double), hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL((k_unpack<Alg, CP, K>), dim3(grid_for(n0, kBlock, 16384)), dim3(kBlock), 0,
-                               ctx->stream, store(), dx, dz, dre, im ? dim : nullptr, (uint32_t)n0);
-            CHK(launch_check());
-        }
-        n_host = (uint32_t)n0;
-        CHK(rebuild_index());
-        hipLaunchKernelGGL(k_index_check<NP>, dim3(grid_for(n_host, kBlock, 16384)), dim3(kBlock), 0, ctx->stream,
-                           ctx->key, ctx->cap, ctx->index, ctx->hslots - 1, ctx->ctrl);
-        CHK(launch_check());
-        CHK(sync_ctrl());
-        if (ctx->ctrl_host->dup) return set_err(ctx, FFX_EINVAL, "observable has duplicate keys");
-        return FFX_OK;
-    }
-
-    int ensure_staging(size_t bytes) {
-        if (bytes <= ctx->staging_bytes) return FFX_OK;
-        dev_free(ctx->staging);
-        ctx->staging_bytes = 0;
-        CHK(dev_alloc(ctx, &ctx->staging, bytes));
-        ctx->staging_bytes = bytes;
-        return FFX_OK;
-    }
-
-    int rotation(const typename Alg::Gate& g, const RotParams<CP>& rp) {
-        CHK(reserve(2 * (size_t)n_host + 64));
-        const unsigned grid = grid_for(n_host, kBlock, 16384);
-        hipLaunchKernelGGL((k_rot<Alg, CP, K>), dim3(grid), dim3(kBlock), 0, ctx->stream, store(), g, rp);
-        CHK(launch_check());
-        std::swap(ctx->coef, ctx->coef2);
-        return sync_ctrl();
-    }
-
-    int clifford(const ffx_pauli_gate* gates, int64_t cb, int64_t ce) {
-        if constexpr (Alg::kPauli) {
-            const size_t ng = (size_t)(ce - cb);
-            if (ng > ctx->cliff_cap) {
-                dev_free(ctx->cliff);
-                CHK(dev_alloc(ctx, &ctx->cliff, ng * sizeof(CliffGate)));
-                ctx->cliff_cap = ng;
-            }
-            std::vector<CliffGate> h(ng);
-            for (size_t t = 0; t < ng; ++t) h[t] = CliffGate{gates[cb + t].kind, gates[cb + t].q0, gates[cb + t].q1, 0};
-            HIPCHK(hipMemcpyAsync(ctx->cliff, h.data(), ng * sizeof(CliffGate), hipMemcpyHostToDevice, ctx->stream));
-            hipLaunchKernelGGL((k_clifford<Alg::W, CP, K>), dim3(grid_for(n_host, kBlock, 16384)), dim3(kBlock), 0,
-                               ctx->stream, store(), ctx->cliff, (int)ng);
-            CHK(launch_check());
-            CHK(rebuild_index());
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-        }
-        return FFX_OK;
-    }
-
-    int expect(const uint64_t* state, double out[2]) {
-        StateWords sw{};
-        if (state)
-            for (int w = 0; w < Alg::W && w < FFX_MAX_WORDS; ++w) sw.s[w] = state[w];
-        const unsigned nb = grid_for(n_host, kBlock, 1024);
-        hipLaunchKernelGGL((k_expect_partial<Alg, CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, store(), sw,
-                           ctx->partial);
-        CHK(launch_check());
-        hipLaunchKernelGGL(k_expect_final, dim3(1), dim3(64), 0, ctx->stream, ctx->partial, (int)nb, ctx->result);
-        CHK(launch_check());
-        HIPCHK(hipMemcpyAsync(ctx->result_host, ctx->result, 2 * sizeof(double), hipMemcpyDeviceToHost,
-                              ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        out[0] = ctx->result_host[0];
-        out[1] = ctx->result_host[1];
-        return FFX_OK;
-    }
-
-    int download(ffx_terms* out) {
-        constexpr int W = Alg::W;
-        std::memset(out, 0, sizeof(*out));
-        out->n_words = W;
-        const size_t nb = (n_host + kTile - 1) / kTile;
-        uint32_t total = 0;
-        if (n_host > 0) {
-            CHK(ensure_counts(nb));
-            Store<CP, K> s = store();
-            hipLaunchKernelGGL((k_count_live<CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, s, ctx->counts);
-            CHK(launch_check());
-            hipLaunchKernelGGL(k_scan_counts, dim3(1), dim3(kBlock), 0, ctx->stream, ctx->counts, (uint32_t)nb,
-                               ctx->total);
-            CHK(launch_check());
-            HIPCHK(hipMemcpyAsync(&total, ctx->total, sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-        }
-        const size_t nt = total;
-        const size_t kb = nt * W * sizeof(uint64_t);
-        out->x = (uint64_t*)malloc(std::max<size_t>(kb, 8));
-        out->z = Alg::kPauli ? (uint64_t*)malloc(std::max<size_t>(kb, 8)) : nullptr;
-        out->c_re = (double*)malloc(std::max<size_t>(nt * sizeof(double), 8));
-        out->c_im = (double*)malloc(std::max<size_t>(nt * sizeof(double), 8));
-        if (!out->x || (Alg::kPauli && !out->z) || !out->c_re || !out->c_im) {
-            ffx_terms_free(out);
-            return set_err(ctx, FFX_ENOMEM, "host allocation of the result failed");
-        }
-        out->n_terms = (int64_t)nt;
-        if (nt == 0) return FFX_OK;
-        const size_t need = (Alg::kPauli ? 2 : 1) * kb + 2 * nt * sizeof(double);
-        CHK(ensure_staging(need));
-        char* sp = (char*)ctx->staging;
-        uint64_t* ox = (uint64_t*)sp;
-        uint64_t* oz = Alg::kPauli ? (uint64_t*)(sp + kb) : nullptr;
-        double* ore = (double*)(sp + (Alg::kPauli ? 2 : 1) * kb);
-        double* oim = ore + nt;
-        hipLaunchKernelGGL((k_pack<Alg, CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, store(), ctx->counts, ox, oz,
-                           ore, oim);
-        CHK(launch_check());
-        HIPCHK(hipMemcpyAsync(out->x, ox, kb, hipMemcpyDeviceToHost, ctx->stream));
-        if (Alg::kPauli) HIPCHK(hipMemcpyAsync(out->z, oz, kb, hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(out->c_re, ore, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipMemcpyAsync(out->c_im, oim, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        return FFX_OK;
-    }
-
-    // ---- asynchronous engine: one launch per step, no host sync between steps ----
-    struct AStep {
-        int kind = 0;  // 0 rotation, 1 Clifford block
-        typename Alg::Gate g{};
-        double cs = 1.0, sn = 0.0;
-        int trunc_thresh = 0, trunc_xw = 0;
-        int compact_after = 0;  // run the (device-decided) compaction sequence after this step
-        int cb = 0, cn = 0;     // Clifford gates [cb, cb+cn) of the block list
-    };
-    uint32_t* ep_buf = nullptr;  // owned by ctx (ctx->epochs)
-    int64_t st_updates = 0, st_outs = 0, st_peak = 0, st_rot = 0, st_compact = 0;
-
-    int ensure_epochs(size_t ne) {
-        if (ne <= ctx->epochs_cap) return FFX_OK;
-        dev_free(ctx->epochs);
-        ctx->epochs_cap = 0;
-        const size_t c = pow2ceil(ne);
-        CHK(dev_alloc(ctx, &ctx->epochs, (4 * c + 64) * sizeof(uint32_t)));
-        ctx->epochs_cap = c;
-        return FFX_OK;
-    }
-
-    Epochs epochs() const {
-        Epochs ep;
-        const size_t c = ctx->epochs_cap;
-        ep.n = ctx->epochs;
-        ep.dead = ctx->epochs + c;
-        ep.app = ctx->epochs + 2 * c;
-        ep.dd = (int32_t*)(ctx->epochs + 3 * c);
-        ep.flags = ctx->epochs + 4 * c;
-        return ep;
-    }
-
-    int run_async(const std::vector<AStep>& steps, const std::vector<CliffGate>& cliff, double mincoeff, int maxdeg,
-                  int max_xw) {
-        if (steps.empty()) return FFX_OK;
-        if (!cliff.empty()) {
-            if (cliff.size() > ctx->cliff_cap) {
-                dev_free(ctx->cliff);
-                CHK(dev_alloc(ctx, &ctx->cliff, cliff.size() * sizeof(CliffGate)));
-                ctx->cliff_cap = cliff.size();
-            }
-            HIPCHK(hipMemcpyAsync(ctx->cliff, cliff.data(), cliff.size() * sizeof(CliffGate), hipMemcpyHostToDevice,
-                                  ctx->stream));
-        }
-        if (ctx->n_cu <= 0) {
-            hipDeviceProp_t prop;
-            HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
-            ctx->n_cu = prop.multiProcessorCount;
-        }
-        int rot_grid = 2 * ctx->n_cu;
-        if (const char* ev = getenv("FFX_ROT_GRID")) rot_grid = std::max(1, atoi(ev));
-        size_t s = 0;
-        for (int attempt = 0; attempt < 64 && s < steps.size(); ++attempt) {
-            // epochs needed from step s on
-            size_t ne = 1;
-            for (size_t t = s; t < steps.size(); ++t) ne += 1 + (steps[t].compact_after ? 1 : 0);
-            CHK(ensure_epochs(ne));
-            CHK(ensure_counts(ctx->cap / kTile + 2));
-            const Epochs ep = epochs();
-            HIPCHK(hipMemsetAsync(ctx->epochs, 0, (4 * ctx->epochs_cap + 64) * sizeof(uint32_t), ctx->stream));
-            uint32_t init[2] = {n_host, dead_host};
-            HIPCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
-            std::vector<int64_t> step_of(ne, -1);
-            std::vector<int> is_rot(ne, 0);
-            std::vector<std::pair<void*, void*>> planes_at(steps.size());
-            const unsigned g_cap = grid_for(ctx->cap, kTile, 4096);
-            const unsigned g_idx = grid_for(std::max(ctx->cap, ctx->hslots), kBlock, 2048);
-            uint32_t e = 1;
-            size_t nev = 0;
-            if (ctx->kernel_timing) {
-                size_t nrot = 0;
-                for (size_t t = s; t < steps.size(); ++t) nrot += steps[t].kind == 0;
-                while (ctx->kev.size() < 2 * nrot) {
-                    hipEvent_t ev;
-                    HIPCHK(hipEventCreate(&ev));
-                    ctx->kev.push_back(ev);
-                }
-            }
-            for (size_t t = s; t < steps.size(); ++t) {
-                const AStep& a = steps[t];
-                planes_at[t] = {ctx->coef, ctx->coef2};
-                if (a.kind == 0) {
-                    if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->kev[2 * nev], ctx->stream));
-                    RotParams<CP> rp{};
-                    rp.cs = make_fac<CP>(a.cs);
-                    rp.sn = make_fac<CP>(a.sn);
-                    rp.mincoeff = mincoeff;
-                    rp.maxdeg = maxdeg;
-                    rp.trunc_thresh = a.trunc_thresh;
-                    rp.trunc_xw = a.trunc_xw;
-                    rp.max_xw = max_xw;
-                    hipLaunchKernelGGL((k_rot_async<Alg, CP, K>), dim3(rot_grid), dim3(kBlockRot), 0, ctx->stream,
-                                       store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, a.g, rp, ep, e);
-                    CHK(launch_check());
-                    if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->kev[2 * nev++ + 1], ctx->stream));
-                    std::swap(ctx->coef, ctx->coef2);
-                    step_of[e] = (int64_t)t;
-                    is_rot[e] = 1;
-                    ++e;
-                } else {
-                    if constexpr (Alg::kPauli) {
-                        hipLaunchKernelGGL((k_cliff_async<Alg::W, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream,
-                                           store(), (const CliffGate*)(ctx->cliff + a.cb), a.cn, ep, e);
-                        CHK(launch_check());
-                        hipLaunchKernelGGL(k_index_build_async<NP>, dim3(g_idx), dim3(kBlock), 0, ctx->stream, ctx->key,
-                                           ctx->cap, ctx->index, ctx->hslots - 1, ep, e, e, 0,
-                                           &ctx->ctrl->index_full);
-                        CHK(launch_check());
-                    }
-                    step_of[e] = (int64_t)t;
-                    ++e;
-                }
-                if (a.compact_after) {
-                    const uint32_t ep_prev = e - 1;
-                    hipLaunchKernelGGL((k_cmp_count<CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream, store(), ep,
-                                       ep_prev, ctx->counts);
-                    CHK(launch_check());
-                    hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(kBlock), 0, ctx->stream, ep, ep_prev, ctx->counts);
-                    CHK(launch_check());
-                    hipLaunchKernelGGL((k_cmp_scatter<NP, CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream, store(),
-                                       ctx->key2, (store_t*)ctx->coef2, ep, ep_prev, e, ctx->counts);
-                    CHK(launch_check());
-                    hipLaunchKernelGGL((k_cmp_copyback<NP, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream, store(),
-                                       ctx->key2, (const store_t*)ctx->coef2, ep, ep_prev, e);
-                    CHK(launch_check());
-                    hipLaunchKernelGGL(k_index_build_async<NP>, dim3(g_idx), dim3(kBlock), 0, ctx->stream, ctx->key,
-                                       ctx->cap, ctx->index, ctx->hslots - 1, ep, ep_prev, e, 1,
-                                       &ctx->ctrl->index_full);
-                    CHK(launch_check());
-                    step_of[e] = (int64_t)t;
-                    ++e;
-                }
-            }
-            // one sync per run: epochs, flags
-            std::vector<uint32_t> hn(e), hd(e), ha(e);
-            std::vector<int32_t> hdd(e);
-            uint32_t hf[2] = {0, 0};
-            HIPCHK(hipMemcpyAsync(hn.data(), ep.n, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(hd.data(), ep.dead, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(ha.data(), ep.app, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(hdd.data(), ep.dd, e * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(hf, ep.flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            for (size_t q = 0; q < nev; ++q) {
-                float ms = 0.f;
-                HIPCHK(hipEventElapsedTime(&ms, ctx->kev[2 * q], ctx->kev[2 * q + 1]));
-                ctx->prof.seconds_kernel += 1e-3 * ms;
-                ctx->prof.run_launches += 1;
-            }
-            const uint32_t e_end = hf[0] ? hf[1] : e;
-            for (uint32_t q = 1; q < e_end; ++q) {
-                if (!is_rot[q]) {
-                    if (hn[q] != hn[q - 1] + ha[q - 1]) st_compact++;
-                    continue;
-                }
-                st_rot++;
-                st_updates += (int64_t)(hn[q] - hd[q]);
-                const int64_t live = (int64_t)(hn[q] + ha[q]) - (int64_t)((int32_t)hd[q] + hdd[q]);
-                st_outs += live;
-                st_peak = std::max<int64_t>(st_peak, live);
-            }
-            if (!hf[0]) {
-                n_host = hn[e - 1] + ha[e - 1];
-                dead_host = (uint32_t)((int32_t)hd[e - 1] + hdd[e - 1]);
-                s = steps.size();
-                break;
-            }
-            // overflow at epoch hf[1]: restore the state at the start of that step and grow
-            const uint32_t ef = hf[1];
-            const size_t tf = (size_t)step_of[ef];
-            ctx->coef = planes_at[tf].first;
-            ctx->coef2 = planes_at[tf].second;
-            n_host = hn[ef];
-            dead_host = hd[ef];
-            {
-                Ctrl c{};
-                c.n = n_host;
-                c.n_dead = dead_host;
-                HIPCHK(hipMemcpyAsync(ctx->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
-            }
-            hipLaunchKernelGGL(k_index_rollback, dim3(grid_for(ctx->hslots, kBlock, 8192)), dim3(kBlock), 0, ctx->stream,
-                               ctx->index, ctx->hslots, n_host);
-            CHK(launch_check());
-            CHK(reserve(2 * ctx->cap));
-            s = tf;
-        }
-        if (s < steps.size()) return set_err(ctx, FFX_ECAPACITY, "term store kept overflowing");
-        HIPCHK(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (ctx->ctrl_host->index_full) return set_err(ctx, FFX_EHIP, "internal error: hash index full");
-        Ctrl c{};
-        c.n = n_host;
-        c.n_dead = dead_host;
-        HIPCHK(hipMemcpyAsync(ctx->ctrl, &c, sizeof(Ctrl), hipMemcpyHostToDevice, ctx->stream));
-        HIPCHK(hipStreamSynchronize(ctx->stream));
-        *ctx->ctrl_host = c;
-        return FFX_OK;
-    }
-
-    void finish_profile() {
-        if (st_rot) {
-            ctx->prof.n_gates = st_rot;
-            ctx->prof.updates = st_updates;
-            ctx->prof.outs = st_outs;
-            ctx->prof.peak_terms = st_peak;
-            ctx->prof.compactions = st_compact;
-        } else {
-            ctx->prof.n_gates = (int64_t)ctx->ctrl_host->n_rot;
-            ctx->prof.updates = (int64_t)ctx->ctrl_host->updates;
-            ctx->prof.outs = (int64_t)ctx->ctrl_host->outs;
-            ctx->prof.peak_terms = (int64_t)ctx->ctrl_host->peak;
-            ctx->prof.compactions = (int64_t)ctx->ctrl_host->n_compact;
-        }
-    }
-};
-
-// FFX_ENGINE=steps selects the one-launch-per-rotation engine with a host sync
-// after every rotation (kept as a cross-check); the default is the asynchronous
-// engine (one launch per step, one sync per propagation).
-bool step_engine() {
-    const char* e = getenv("FFX_ENGINE");
-    return e && std::strcmp(e, "steps") == 0;
-}
-
-template <int W>
-typename PauliAlg<W>::Gate pauli_gate_dev(const ffx_pauli_gate& g) {
-    typename PauliAlg<W>::Gate d{};
-    int pxz = 0;
-    for (int w = 0; w < W; ++w) {
-        d.p[w] = g.x[w];
-        d.p[W + w] = g.z[w];
-        pxz += __builtin_popcountll(g.x[w] & g.z[w]);
-    }
-    d.pxz = pxz;
-    return d;
-}
-
-template <int W>
-typename MajoranaAlg<W>::Gate majorana_gate_dev(const ffx_majorana_gate& g) {
-    typename MajoranaAlg<W>::Gate d{};
-    int deg = 0;
-    for (int w = 0; w < W; ++w) {
-        d.p[w] = g.m[w];
-        deg += __builtin_popcountll(g.m[w]);
-    }
-    // suffix parity: bit t set iff an odd number of axis bits lie strictly above t
-    int par = 0;
-    for (int t = 64 * W - 1; t >= 0; --t) {
-        if (par) d.sm[t >> 6] |= 1ull << (t & 63);
-        if ((g.m[t >> 6] >> (t & 63)) & 1ull) par ^= 1;
-    }
-    d.deg = deg;
-    d.r = g.r;
-    return d;
-}
-
-template <int W, class CP, int K>
-int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const uint64_t* x, const uint64_t* z,
-              const double* re, const double* im, int64_t n_terms, const ffx_options& o, ffx_terms* out,
-              const uint64_t* state, double* out_expect) {
-    using Alg = PauliAlg<W>;
-    Engine<Alg, CP, K> eng(ctx);
-    const double t0 = now_s();
-    CHK(eng.upload(x, z, re, im, n_terms));
-    const double t1 = now_s();
-    const std::vector<Step> steps = pauli_schedule<W>(gates, n_gates, o.batched != 0);
-    const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
-    if (o.topk > 0) return set_err(ctx, FFX_EINVAL, "topk truncation is not supported on the device path");
-    if (!step_engine()) {
-        using AStep = typename Engine<Alg, CP, K>::AStep;
-        std::vector<AStep> sd;
-        std::vector<CliffGate> cl;
-        sd.reserve(steps.size());
-        int trunc_points = 0;
-        for (const Step& s : steps) {
-            AStep d{};
-            if (s.kind == 0) {
-                const ffx_pauli_gate& g = gates[s.gate];
-                d.g = pauli_gate_dev<W>(g);
-                gate_trig(g.theta, d.cs, d.sn);
-                d.kind = 0;
-                d.trunc_thresh = (s.trunc && o.mincoeff > 0) ? 1 : 0;
-                d.trunc_xw = (s.trunc && o.max_xweight >= 0 && o.xtrunc_period > 0 &&
-                              (s.rc + 1) % o.xtrunc_period == 0)
-                                 ? 1
-                                 : 0;
-                // batched runs compact at every batch end; per-gate runs every 16th truncation
-                if (s.trunc) d.compact_after = (o.batched || (++trunc_points % 16) == 0) ? 1 : 0;
-            } else {
-                d.kind = 1;
-                d.cb = (int)cl.size();
-                d.cn = (int)(s.ce - s.cb);
-                for (int64_t t = s.cb; t < s.ce; ++t) cl.push_back(CliffGate{gates[t].kind, gates[t].q0, gates[t].q1, 0});
-            }
-            sd.push_back(d);
-        }
-        CHK(eng.run_async(sd, cl, o.mincoeff, maxdeg, o.max_xweight));
-    } else
-    for (const Step& s : steps) {
-        if (s.kind == 0) {
-            const ffx_pauli_gate& g = gates[s.gate];
-            RotParams<CP> rp{};
-            double cs_, sn_;
-            gate_trig(g.theta, cs_, sn_);
-            rp.cs = make_fac<CP>(cs_);
-            rp.sn = make_fac<CP>(sn_);
-            rp.maxdeg = maxdeg;
-            rp.mincoeff = o.mincoeff;
-            rp.trunc_thresh = (s.trunc && o.mincoeff > 0) ? 1 : 0;
-            rp.trunc_xw = (s.trunc && o.max_xweight >= 0 && o.xtrunc_period > 0 &&
-                           (s.rc + 1) % o.xtrunc_period == 0 && o.topk <= 0)
-                              ? 1
-                              : 0;
-            rp.max_xw = o.max_xweight;
-            CHK(eng.rotation(pauli_gate_dev<W>(g), rp));
-            if (s.trunc && o.topk > 0)
-                return set_err(ctx, FFX_EINVAL, "topk truncation is not supported on the device path");
-            if (s.trunc && eng.dead_host > 0 && (size_t)eng.dead_host * 4 > eng.n_host) CHK(eng.compact());
-        } else {
-            CHK(eng.clifford(gates, s.cb, s.ce));
-        }
-    }
-    CHK(eng.sync_ctrl());
-    eng.finish_profile();
-    const double t2 = now_s();
-    if (out_expect) CHK(eng.expect(state, out_expect));
-    if (out) CHK(eng.download(out));
-    const double t3 = now_s();
-    ctx->prof.seconds_upload = t1 - t0;
-    ctx->prof.seconds_propagate = t2 - t1;
-    ctx->prof.seconds_download = t3 - t2;
-    ctx->prof.seconds_total = t3 - t0;
-    ctx->last_kind = 1;
-    ctx->last_W = W;
-    ctx->last_K = K;
-    ctx->last_prec = std::is_same<CP, F64>::value ? FFX_FP64 : (std::is_same<CP, F32>::value ? FFX_FP32 : FFX_FP32C);
-    return FFX_OK;
-}
-
-template <int W, class CP, int K>
-int run_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, const uint64_t* m, const double* re,
-                 const double* im, int64_t n_terms, int maxdegree, double mincoeff, ffx_terms* out,
-                 const uint64_t* occ, double* out_expect) {
-    using Alg = MajoranaAlg<W>;
-    Engine<Alg, CP, K> eng(ctx);
-    const double t0 = now_s();
-    CHK(eng.upload(m, nullptr, re, im, n_terms));
-    const double t1 = now_s();
-    const int maxdeg = maxdegree < 0 ? (1 << 30) : maxdegree;
-    if (!step_engine()) {
-        using AStep = typename Engine<Alg, CP, K>::AStep;
-        std::vector<AStep> sd;
-        sd.reserve((size_t)n_gates);
-        int cnt = 0;
-        for (int64_t i = n_gates - 1; i >= 0; --i) {
-            AStep d{};
-            d.g = majorana_gate_dev<W>(gates[i]);
-            gate_trig(gates[i].theta, d.cs, d.sn);
-            d.kind = 0;
-            d.trunc_thresh = mincoeff > 0 ? 1 : 0;
-            d.compact_after = (mincoeff > 0 && (++cnt % 16) == 0) ? 1 : 0;
-            sd.push_back(d);
-        }
-        CHK(eng.run_async(sd, {}, mincoeff, maxdeg, -1));
-    } else
-    for (int64_t i = n_gates - 1; i >= 0; --i) {
-        const ffx_majorana_gate& g = gates[i];
-        RotParams<CP> rp{};
-        double cs_, sn_;
-        gate_trig(g.theta, cs_, sn_);
-        rp.cs = make_fac<CP>(cs_);
-        rp.sn = make_fac<CP>(sn_);
-        rp.maxdeg = maxdeg;
-        rp.mincoeff = mincoeff;
-        rp.trunc_thresh = mincoeff > 0 ? 1 : 0;
-        CHK(eng.rotation(majorana_gate_dev<W>(g), rp));
-        if (eng.dead_host > 0 && (size_t)eng.dead_host * 4 > eng.n_host) CHK(eng.compact());
-    }
-    CHK(eng.sync_ctrl());
-    eng.finish_profile();
-    const double t2 = now_s();
-    if (out_expect) CHK(eng.expect(occ, out_expect));
-    if (out) CHK(eng.download(out));
-    const double t3 = now_s();
-    ctx->prof.seconds_upload = t1 - t0;
-    ctx->prof.seconds_propagate = t2 - t1;
-    ctx->prof.seconds_download = t3 - t2;
-    ctx->prof.seconds_total = t3 - t0;
-    ctx->last_kind = 2;
-    ctx->last_W = W;
-    ctx->last_K = K;
-    ctx->last_prec = std::is_same<CP, F64>::value ? FFX_FP64 : (std::is_same<CP, F32>::value ? FFX_FP32 : FFX_FP32C);
-    return FFX_OK;
-}
-
-template <class Alg, class CP, int K>
-int expect_last_impl(ffx_ctx* ctx, const uint64_t* state, double* out) {
-    Engine<Alg, CP, K> eng(ctx);
-    CHK(eng.sync_ctrl());
-    return eng.expect(state, out);
-}
-
-int mono_degree_mod4_r(const uint64_t* m, int W) {
-    int d = 0;
-    for (int w = 0; w < W; ++w) d += __builtin_popcountll(m[w]);
-    return (d >> 1) & 1;
-}
-
-}  // namespace
+namespace ffx_host {
+FFX_DISPATCH_DECL(extern, 1);
+FFX_DISPATCH_DECL(extern, 2);
+FFX_DISPATCH_DECL(extern, 3);
+FFX_DISPATCH_DECL(extern, 4);
+}  // namespace ffx_host
 
 // ---------------------------------------------------------------------------
 // C ABI
@@ -1014,9 +89,10 @@ void ffx_terms_free(ffx_terms* t) {
     t->n_terms = 0;
 }
 
-int ffx_set_kernel_timing(ffx_ctx* ctx, int on) {
+int ffx_set_kernel_timing(ffx_ctx* ctx, int every) {
     if (!ctx) return FFX_EINVAL;
-    ctx->kernel_timing = on ? 1 : 0;
+    if (every < 0) return set_err(ctx, FFX_EINVAL, "kernel timing period must be >= 0");
+    ctx->kernel_timing = every;
     return FFX_OK;
 }
 
@@ -1063,25 +139,12 @@ int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gat
     ctx->launches = 0;
     const int prec = opts->precision;
     if (prec < FFX_FP64 || prec > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
-#define FFX_RUNP(W_, CP_, K_)                                                                               \
-    return run_pauli<W_, CP_, K_>(ctx, gates, n_gates, x, z, c_re, K_ == 2 ? c_im : nullptr, n_terms, *opts, \
-                                  out, basis_state, out_expect)
-#define FFX_RUNP_K(W_, CP_)        \
-    if (K == 1) FFX_RUNP(W_, CP_, 1); \
-    FFX_RUNP(W_, CP_, 2)
-#define FFX_RUNP_PREC(W_)                            \
-    if (prec == FFX_FP64) { FFX_RUNP_K(W_, F64); }   \
-    if (prec == FFX_FP32) { FFX_RUNP_K(W_, F32); }   \
-    FFX_RUNP_K(W_, F32C)
     switch (n_words) {
-        case 1: FFX_RUNP_PREC(1);
-        case 2: FFX_RUNP_PREC(2);
-        case 3: FFX_RUNP_PREC(3);
-        default: FFX_RUNP_PREC(4);
+        case 1: return dispatch_pauli<1>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
+        case 2: return dispatch_pauli<2>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
+        case 3: return dispatch_pauli<3>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
+        default: return dispatch_pauli<4>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
     }
-#undef FFX_RUNP_PREC
-#undef FFX_RUNP_K
-#undef FFX_RUNP
 }
 
 int ffx_majorana_propagate(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,
@@ -1108,25 +171,12 @@ int ffx_majorana_propagate(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t
     ctx->prof = ffx_profile{};
     ctx->launches = 0;
     if (precision < FFX_FP64 || precision > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
-#define FFX_RUNM(W_, CP_, K_)                                                                                 \
-    return run_majorana<W_, CP_, K_>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, \
-                                     out_expect)
-#define FFX_RUNM_K(W_, CP_)        \
-    if (K == 1) FFX_RUNM(W_, CP_, 1); \
-    FFX_RUNM(W_, CP_, 2)
-#define FFX_RUNM_PREC(W_)                                 \
-    if (precision == FFX_FP64) { FFX_RUNM_K(W_, F64); }   \
-    if (precision == FFX_FP32) { FFX_RUNM_K(W_, F32); }   \
-    FFX_RUNM_K(W_, F32C)
     switch (n_words) {
-        case 1: FFX_RUNM_PREC(1);
-        case 2: FFX_RUNM_PREC(2);
-        case 3: FFX_RUNM_PREC(3);
-        default: FFX_RUNM_PREC(4);
+        case 1: return dispatch_majorana<1>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K);
+        case 2: return dispatch_majorana<2>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K);
+        case 3: return dispatch_majorana<3>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K);
+        default: return dispatch_majorana<4>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K);
     }
-#undef FFX_RUNM_PREC
-#undef FFX_RUNM_K
-#undef FFX_RUNM
 }
 
 int ffx_expect_last(ffx_ctx* ctx, const uint64_t* state, double out[2]) {
@@ -1135,29 +185,12 @@ int ffx_expect_last(ffx_ctx* ctx, const uint64_t* state, double out[2]) {
     if (ctx->last_kind == 0) return set_err(ctx, FFX_EINVAL, "no device-resident result");
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, FFX_ENODEV, "hipSetDevice failed");
     const int W = ctx->last_W, K = ctx->last_K, prec = ctx->last_prec;
-#define FFX_EX(ALG_, W_, CP_, K_) return expect_last_impl<ALG_<W_>, CP_, K_>(ctx, state, out)
-#define FFX_EX_K(ALG_, W_, CP_)   \
-    if (K == 1) FFX_EX(ALG_, W_, CP_, 1); \
-    FFX_EX(ALG_, W_, CP_, 2)
-#define FFX_EX_P(ALG_, W_)                             \
-    if (prec == FFX_FP64) { FFX_EX_K(ALG_, W_, F64); } \
-    if (prec == FFX_FP32) { FFX_EX_K(ALG_, W_, F32); } \
-    FFX_EX_K(ALG_, W_, F32C)
-#define FFX_EX_W(ALG_)              \
-    switch (W) {                    \
-        case 1: FFX_EX_P(ALG_, 1);  \
-        case 2: FFX_EX_P(ALG_, 2);  \
-        case 3: FFX_EX_P(ALG_, 3);  \
-        default: FFX_EX_P(ALG_, 4); \
+    switch (W) {
+        case 1: return dispatch_expect<1>(ctx, state, out, ctx->last_kind, prec, K);
+        case 2: return dispatch_expect<2>(ctx, state, out, ctx->last_kind, prec, K);
+        case 3: return dispatch_expect<3>(ctx, state, out, ctx->last_kind, prec, K);
+        default: return dispatch_expect<4>(ctx, state, out, ctx->last_kind, prec, K);
     }
-    if (ctx->last_kind == 1) {
-        FFX_EX_W(PauliAlg);
-    }
-    FFX_EX_W(MajoranaAlg);
-#undef FFX_EX_W
-#undef FFX_EX_P
-#undef FFX_EX_K
-#undef FFX_EX
 }
 
 }  // extern "C"

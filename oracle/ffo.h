@@ -89,6 +89,10 @@ void ffo_majorana_expect(const ffo_terms* t, const uint64_t* occ, double out[2])
 
 void ffo_free(ffo_terms* t);
 
+/* Live terms entering each rotation of the last propagate call, in application
+ * order (serial strategy); returns the rotation count, writes at most cap. */
+int64_t ffo_rotation_counts(int64_t* out, int64_t cap);
+
 /* Threads the OpenMP runtime would use by default (0 when built without OpenMP). */
 int32_t ffo_max_threads(void);
 

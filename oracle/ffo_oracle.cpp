@@ -41,9 +41,11 @@ using cplx = std::complex<double>;
 struct Stats {
     int64_t updates = 0;
     int64_t peak = 0;
+    std::vector<int64_t> n_in_per_rotation;  // live terms entering each rotation, in application order
     void gate(int64_t n_in, int64_t n_out) {
         updates += n_in;
         if (n_out > peak) peak = n_out;
+        n_in_per_rotation.push_back(n_in);
     }
 };
 Stats g_stats;
@@ -893,6 +895,12 @@ void ffo_free(ffo_terms* t) {
     t->x = t->z = nullptr;
     t->c_re = t->c_im = nullptr;
     t->n_terms = 0;
+}
+
+int64_t ffo_rotation_counts(int64_t* out, int64_t cap) {
+    const int64_t n = (int64_t)g_stats.n_in_per_rotation.size();
+    for (int64_t i = 0; i < n && i < cap; ++i) out[i] = g_stats.n_in_per_rotation[i];
+    return n;
 }
 
 int32_t ffo_max_threads(void) {

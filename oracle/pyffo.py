@@ -65,6 +65,8 @@ def lib():
         L.ffo_pauli_expect.argtypes = [ctypes.POINTER(_Terms), vp, vp]
         L.ffo_majorana_expect.argtypes = [ctypes.POINTER(_Terms), vp, vp]
         L.ffo_free.argtypes = [ctypes.POINTER(_Terms)]
+        L.ffo_rotation_counts.argtypes = [ctypes.c_void_p, ctypes.c_int64]
+        L.ffo_rotation_counts.restype = ctypes.c_int64
         L.ffo_max_threads.restype = ctypes.c_int32
         _LIB = L
     return _LIB
@@ -205,3 +207,12 @@ def majorana_expect(res: Result, occ_words) -> complex:
 
 def max_threads() -> int:
     return int(lib().ffo_max_threads())
+
+
+def rotation_counts():
+    """Live terms entering each rotation of the last propagate call."""
+    L = lib()
+    n = L.ffo_rotation_counts(None, 0)
+    out = np.zeros(n, np.int64)
+    L.ffo_rotation_counts(out.ctypes.data, n)
+    return out
