@@ -98,6 +98,14 @@ def main():
     bytes_per_launch = alg_bytes / max(kernel_launches, 1)
     achieved = bytes_per_launch / kernel_avg if kernel_avg > 0 else 0.0
 
+    # HBM bytes per launch of the same kernel and workload, from the two rocprofv3
+    # --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled against a plain copy kernel
+    traffic = None
+    tf = os.path.join(ROOT, "profiles", "k_win_async_kicked_ising_10x10_fp64_traffic.json")
+    if (L, layers, a.batched, a.precision) == (10, 20, 1, "fp64") and os.path.exists(tf):
+        with open(tf) as fh:
+            traffic = json.load(fh)["bytes_per_launch"]
+
     cpu = None
     oracle_expect = None
     if not a.no_cpu:
@@ -145,7 +153,7 @@ def main():
             "peak": HBM_PEAK / 1e9,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
-            "traffic": None,
+            "traffic": traffic,
             "kernel": "k_win_async / k_rot_async (one window of up to 4 commuting rotations, or one rotation, "
                       "per launch)",
             "kernel_ms_per_launch": 1e3 * kernel_avg,

@@ -636,7 +636,7 @@ struct Engine {
         int rot_grid = 2 * ctx->n_cu;
         if (const char* ev = getenv("FFX_ROT_GRID"))
             if (atoi(ev) > 0) rot_grid = atoi(ev);
-        int win_grid = 16 * ctx->n_cu;
+        int win_grid = 6 * ctx->n_cu;
         if (const char* ev = getenv("FFX_WIN_GRID"))
             if (atoi(ev) > 0) win_grid = atoi(ev);
         int max_window = kWin;

@@ -358,15 +358,13 @@ __global__ void __launch_bounds__(kBlockWin) k_win_async(Store<CP, K> st, const 
     uint32_t n, dead;
     if (!step_enter(ep, e, n, dead)) return;
     const size_t cap = st.cap;
-    const uint32_t per = (n + gridDim.x - 1) / gridDim.x;
-    const uint32_t chunk = (per + blockDim.x - 1) / blockDim.x * blockDim.x;
-    const uint32_t begin = blockIdx.x * chunk;
-    const uint32_t end = min(n, begin + chunk);
+    // block-strided tiles of blockDim slots
+    const uint32_t begin = blockIdx.x * blockDim.x, end = n, step = gridDim.x * blockDim.x;
     if (begin >= end) return;  // block-uniform: nothing to do, nothing to report
     __shared__ uint32_t sbase;
     int dd = 0;
     int sdl[kWin] = {0, 0, 0, 0};
-    for (uint32_t b0 = begin; b0 < end; b0 += blockDim.x) {
+    for (uint32_t b0 = begin; b0 < end; b0 += step) {
         const uint32_t i = b0 + threadIdx.x;
         uint32_t fresh = 0;
         uint64_t k[NP];
