@@ -291,6 +291,124 @@ __global__ void __launch_bounds__(kBlock) k_cliff_async(Store<CP, K> st, const C
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i <= st.hmask; i += stride) st.index[i] = kEmpty;
 }
 
+// ---- top-k truncation (src/pauli/truncate.h:21-39), device-side ----
+// Keeps the topk largest |c| of the live terms: a radix select over the bits of
+// |c| (a non-negative double orders like its bit pattern) finds V, the
+// (topk+1)-th largest magnitude, 8 bits per pass from the top; then every live
+// term below V is erased, and of the terms equal to V only as many as needed to
+// keep exactly topk (the reference erases the surplus ties in map order, which
+// is not reproducible; here the survivors among ties are arbitrary).
+struct TopK {
+    uint32_t* hist;              // 256 bins of the current digit
+    unsigned long long* prefix;  // magnitude bits selected so far
+    uint32_t* want;              // rank of the target among the terms matching the prefix
+    uint32_t* tiec;              // tie tickets
+};
+
+template <class CP, int K>
+__device__ __forceinline__ uint64_t mag_bits(const Store<CP, K>& st, size_t i) {
+    typename CP::val_t v[K];
+#pragma unroll
+    for (int kk = 0; kk < K; ++kk) v[kk] = st.ld(st.coef, kk, i);
+    return (uint64_t)__double_as_longlong(magnitude<CP, K>(v));
+}
+
+__device__ __forceinline__ void state_after(const Epochs& ep, uint32_t e_prev, uint32_t& n, uint32_t& live) {
+    n = ep.n[e_prev] + ep.app[e_prev];
+    live = n - (uint32_t)((int32_t)ep.dead[e_prev] + ep.dd[e_prev]);
+}
+
+template <class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_topk_hist(Store<CP, K> st, Epochs ep, uint32_t e_prev, TopK tk,
+                                                      uint32_t topk, int shift) {
+    if (run_stopped(ep)) return;
+    uint32_t n, live;
+    state_after(ep, e_prev, n, live);
+    if (live <= topk) return;
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = 0;
+    __syncthreads();
+    const uint64_t mask = shift >= 56 ? 0ull : (~0ull << (shift + 8));
+    const uint64_t pre = shift >= 56 ? 0ull : (uint64_t)*tk.prefix;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (CP::dead(st.ld(st.coef, 0, i))) continue;
+        const uint64_t b = mag_bits<CP, K>(st, i);
+        if ((b & mask) == (pre & mask)) atomicAdd(&h[(b >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (h[threadIdx.x]) atomicAdd(&tk.hist[threadIdx.x], h[threadIdx.x]);
+}
+
+// One block of 256 threads: choose the digit, update prefix and rank, clear the bins.
+static __global__ void __launch_bounds__(kBlock) k_topk_pick(Epochs ep, uint32_t e_prev, TopK tk, uint32_t topk,
+                                                      int shift) {
+    if (run_stopped(ep)) return;
+    uint32_t n, live;
+    state_after(ep, e_prev, n, live);
+    if (live <= topk) return;
+    __shared__ uint32_t h[256];
+    h[threadIdx.x] = tk.hist[threadIdx.x];
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t want = shift >= 56 ? topk : *tk.want;
+        uint64_t pre = shift >= 56 ? 0ull : (uint64_t)*tk.prefix;
+        uint32_t cum = 0;
+        int digit = 0;
+        for (int b = 255; b >= 0; --b) {
+            if (cum + h[b] > want) {
+                digit = b;
+                break;
+            }
+            cum += h[b];
+        }
+        *tk.prefix = (unsigned long long)(pre | ((uint64_t)digit << shift));
+        *tk.want = want - cum;
+        *tk.tiec = 0;
+    }
+    tk.hist[threadIdx.x] = 0;
+}
+
+// Epoch e: erase below the selected magnitude and the surplus ties, then (xw)
+// the X/Y-weight cap, which the reference applies after top-k (truncate.h:54-56).
+template <class Alg, class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_topk_apply(Store<CP, K> st, Epochs ep, uint32_t e, TopK tk,
+                                                       uint32_t topk, int xw, int max_xw) {
+    constexpr int NP = Alg::NP;
+    uint32_t n, dead;
+    if (!step_enter(ep, e, n, dead)) return;
+    const bool sel = n - dead > topk;
+    if (!sel && !xw) return;
+    const uint64_t V = sel ? (uint64_t)*tk.prefix : 0ull;
+    const uint32_t keep_ties = sel ? *tk.want : 0u;
+    const size_t cap = st.cap;
+    int dd = 0;
+    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        if (CP::dead(st.ld(st.coef, 0, i))) continue;
+        bool kill = false;
+        if (sel) {
+            const uint64_t b = mag_bits<CP, K>(st, i);
+            if (b < V)
+                kill = true;
+            else if (b == V)
+                kill = atomicAdd(tk.tiec, 1u) >= keep_ties;
+        }
+        if (!kill && xw) {
+            uint64_t k[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
+            kill = Alg::xweight(k) > max_xw;
+        }
+        if (kill) {
+            st.st(st.coef, 0, i, CP::make_dead());
+#pragma unroll
+            for (int kk = 1; kk < K; ++kk) st.st(st.coef, kk, i, CP::from_double(0.0));
+            ++dd;
+        }
+    }
+    dd = block_sum_int(dd);
+    if (threadIdx.x == 0 && dd) atomicAdd(&ep.dd[e], dd);
+}
+
 // Remove index entries of slots >= n (the appends of a step that overflowed).
 static __global__ void k_index_rollback(uint64_t* index, size_t nslots, uint32_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < nslots; i += (size_t)gridDim.x * blockDim.x) {

@@ -53,6 +53,7 @@ struct ffx_ctx {
     CliffGate* cliff = nullptr;
     size_t cliff_cap = 0;
     uint32_t* epochs = nullptr;
+    uint32_t* topk_buf = nullptr;  // top-k select scratch: 256 bins, prefix, rank, tie tickets
     size_t epochs_cap = 0;
     int n_cu = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -537,6 +538,8 @@ struct Engine {
         double cs = 1.0, sn = 0.0;
         int trunc_thresh = 0, trunc_xw = 0;
         int compact_after = 0;  // run the (device-decided) compaction sequence after this step
+        int topk_after = 0;     // top-k truncation after this step
+        int topk_xw = 0;        // ... followed by the X/Y-weight cap
         int cb = 0, cn = 0;     // Clifford gates [cb, cb+cn) of the block list
     };
     uint32_t* ep_buf = nullptr;  // owned by ctx (ctx->epochs)
@@ -607,7 +610,7 @@ struct Engine {
         size_t k = 1;
         while ((int)k < max_window && t + k < steps.size()) {
             const AStep& nx = steps[t + k];
-            if (steps[t + k - 1].compact_after || nx.kind != 0) break;
+            if (steps[t + k - 1].compact_after || steps[t + k - 1].topk_after || nx.kind != 0) break;
             bool ok = true;
             for (size_t j = 0; j < k && ok; ++j) ok = axes_commute(steps[t + j].g, nx.g);
             if (!ok || !add(nx.g.p)) break;
@@ -616,8 +619,17 @@ struct Engine {
         return k;
     }
 
+    TopK topk_scratch() const {
+        TopK tk;
+        tk.hist = ctx->topk_buf;
+        tk.prefix = (unsigned long long*)(ctx->topk_buf + 256);
+        tk.want = ctx->topk_buf + 258;
+        tk.tiec = ctx->topk_buf + 259;
+        return tk;
+    }
+
     int run_async(const std::vector<AStep>& steps, const std::vector<CliffGate>& cliff, double mincoeff, int maxdeg,
-                  int max_xw) {
+                  int max_xw, uint32_t topk = 0) {
         if (steps.empty()) return FFX_OK;
         if (!cliff.empty()) {
             if (cliff.size() > ctx->cliff_cap) {
@@ -645,11 +657,16 @@ struct Engine {
         for (int attempt = 0; attempt < 64 && s < steps.size(); ++attempt) {
             // epochs needed from step s on
             size_t ne = 1;
-            for (size_t t = s; t < steps.size(); ++t) ne += 1 + (steps[t].compact_after ? 1 : 0);
+            for (size_t t = s; t < steps.size(); ++t)
+                ne += 1 + (steps[t].compact_after ? 1 : 0) + (steps[t].topk_after ? 1 : 0);
             CHK(ensure_epochs(ne));
             CHK(ensure_counts(ctx->cap / kTile + 2));
             const Epochs ep = epochs();
             HIPCHK(hipMemsetAsync(ctx->epochs, 0, epoch_words(ctx->epochs_cap) * sizeof(uint32_t), ctx->stream));
+            if (topk > 0) {
+                if (!ctx->topk_buf) CHK(dev_alloc(ctx, &ctx->topk_buf, 264 * sizeof(uint32_t)));
+                HIPCHK(hipMemsetAsync(ctx->topk_buf, 0, 264 * sizeof(uint32_t), ctx->stream));
+            }
             uint32_t init[2] = {n_host, dead_host};
             HIPCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
             HIPCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
@@ -732,6 +749,24 @@ struct Engine {
                                            &ctx->ctrl->index_full);
                         CHK(launch_check());
                     }
+                    step_of[e] = (int64_t)t;
+                    ++e;
+                }
+                if (steps[t + kw - 1].topk_after) {
+                    const uint32_t ep_prev = e - 1;
+                    const TopK tk = topk_scratch();
+                    const unsigned g_sel = grid_for(ctx->cap, 4 * kBlock, 1024);
+                    for (int shift = 56; shift >= 0; shift -= 8) {
+                        hipLaunchKernelGGL((k_topk_hist<CP, K>), dim3(g_sel), dim3(kBlock), 0, ctx->stream, store(), ep,
+                                           ep_prev, tk, topk, shift);
+                        CHK(launch_check());
+                        hipLaunchKernelGGL(k_topk_pick, dim3(1), dim3(kBlock), 0, ctx->stream, ep, ep_prev, tk, topk,
+                                           shift);
+                        CHK(launch_check());
+                    }
+                    hipLaunchKernelGGL((k_topk_apply<Alg, CP, K>), dim3(g_sel), dim3(kBlock), 0, ctx->stream, store(), ep,
+                                       e, tk, topk, steps[t + kw - 1].topk_xw, max_xw);
+                    CHK(launch_check());
                     step_of[e] = (int64_t)t;
                     ++e;
                 }
@@ -907,7 +942,6 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
     const double t1 = now_s();
     const std::vector<Step> steps = pauli_schedule<W>(gates, n_gates, o.batched != 0);
     const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
-    if (o.topk > 0) return set_err(ctx, FFX_EINVAL, "topk truncation is not supported on the device path");
     if (!step_engine()) {
         using AStep = typename Engine<Alg, CP, K>::AStep;
         std::vector<AStep> sd;
@@ -926,6 +960,12 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
                               (s.rc + 1) % o.xtrunc_period == 0)
                                  ? 1
                                  : 0;
+                if (s.trunc && o.topk > 0) {
+                    // threshold, then top-k, then the X/Y cap (truncate.h:51-57)
+                    d.topk_after = 1;
+                    d.topk_xw = d.trunc_xw;
+                    d.trunc_xw = 0;
+                }
                 // batched runs compact at every batch end; per-gate runs every 16th truncation
                 if (s.trunc) d.compact_after = (o.batched || (++trunc_points % 16) == 0) ? 1 : 0;
             } else {
@@ -936,7 +976,7 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
             }
             sd.push_back(d);
         }
-        CHK(eng.run_async(sd, cl, o.mincoeff, maxdeg, o.max_xweight));
+        CHK(eng.run_async(sd, cl, o.mincoeff, maxdeg, o.max_xweight, o.topk > 0 ? (uint32_t)o.topk : 0u));
     } else
     for (const Step& s : steps) {
         if (s.kind == 0) {

@@ -66,6 +66,7 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     dev_free(ctx->staging);
     dev_free(ctx->cliff);
     dev_free(ctx->epochs);
+    dev_free(ctx->topk_buf);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
