@@ -23,7 +23,7 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP error",
 
 EXPORTED = ("ffx_ctx_create", "ffx_ctx_destroy", "ffx_last_error", "ffx_pauli_propagate",
             "ffx_majorana_propagate", "ffx_expect_last", "ffx_terms_free", "ffx_profile_get",
-            "ffx_device_count", "ffx_set_kernel_timing")
+            "ffx_device_count", "ffx_set_kernel_timing", "ffx_pauli_propagate_batch")
 
 
 class Options(ctypes.Structure):
@@ -37,6 +37,12 @@ class Terms(ctypes.Structure):
     _fields_ = [("n_terms", ctypes.c_int64), ("n_words", ctypes.c_int32), ("reserved", ctypes.c_int32),
                 ("x", ctypes.POINTER(ctypes.c_uint64)), ("z", ctypes.POINTER(ctypes.c_uint64)),
                 ("c_re", ctypes.POINTER(ctypes.c_double)), ("c_im", ctypes.POINTER(ctypes.c_double))]
+
+
+class Instance(ctypes.Structure):
+    _fields_ = [("gates", ctypes.c_void_p), ("n_gates", ctypes.c_int64), ("x", ctypes.c_void_p),
+                ("z", ctypes.c_void_p), ("c_re", ctypes.c_void_p), ("c_im", ctypes.c_void_p),
+                ("n_terms", ctypes.c_int64), ("basis_state", ctypes.c_uint64 * 4)]
 
 
 class Profile(ctypes.Structure):
@@ -75,6 +81,8 @@ def load():
             L.ffx_profile_get.argtypes = [vp, ctypes.POINTER(Profile)]
             L.ffx_device_count.argtypes = []
             L.ffx_set_kernel_timing.argtypes = [vp, ctypes.c_int]
+            L.ffx_pauli_propagate_batch.argtypes = [vp, ctypes.POINTER(Instance), i32, i32, ctypes.POINTER(Options),
+                                                    i64, vp, vp, vp]
             _lib = L
     return _lib
 
@@ -168,6 +176,36 @@ class Context:
         if want_expect or state is not None:
             return res, complex(ex[0], ex[1])
         return res
+
+    def pauli_propagate_batch(self, instances, n_words, maxdegree=-1, mincoeff=0.0, topk=0, max_xweight=-1,
+                              xtrunc_period=1, batched=True, precision=FP64, slots=0):
+        """instances: iterable of (gates, x, z, c, state) with state a list of
+        up to 4 uint64 words (or None for |0...0>).  Returns (expectations
+        complex[n], live term counts int64[n], number of instances that ran on
+        the device-wide engine)."""
+        keep = []
+        arr = (Instance * max(len(instances), 1))()
+        for i, (gates, x, z, c, state) in enumerate(instances):
+            gates = np.ascontiguousarray(gates, dtype=PAULI_GATE_DTYPE)
+            x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, n_words)
+            z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, n_words)
+            c = np.asarray(c, dtype=np.complex128).reshape(-1)
+            re = np.ascontiguousarray(c.real)
+            im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+            keep += [gates, x, z, re, im]
+            arr[i].gates, arr[i].n_gates = _p(gates), len(gates)
+            arr[i].x, arr[i].z, arr[i].c_re, arr[i].c_im = _p(x), _p(z), _p(re), _p(im)
+            arr[i].n_terms = len(c)
+            for w, v in enumerate(list(state or [])[:4]):
+                arr[i].basis_state[w] = int(v)
+        n = len(instances)
+        opts = Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        ex = np.zeros(2 * max(n, 1))
+        nt = np.zeros(max(n, 1), np.int64)
+        fb = ctypes.c_int32(0)
+        self._check(self._L.ffx_pauli_propagate_batch(self._h, arr, n, n_words, ctypes.byref(opts), slots, _p(ex),
+                                                      _p(nt), ctypes.byref(fb)))
+        return ex[0:2 * n:2] + 1j * ex[1:2 * n:2], nt[:n], int(fb.value)
 
     def majorana_propagate(self, gates, n_words, m, c, maxdegree=-1, mincoeff=0.0, precision=FP64,
                            download=True, occ=None, want_expect=False):

@@ -23,7 +23,7 @@
 #include <vector>
 
 #include "../../include/fastfermion_amd.h"
-#include "ffx_window.h"
+#include "ffx_batch.h"
 
 using namespace ffx;
 
@@ -54,6 +54,8 @@ struct ffx_ctx {
     size_t cliff_cap = 0;
     uint32_t* epochs = nullptr;
     uint32_t* topk_buf = nullptr;  // top-k select scratch: 256 bins, prefix, rank, tie tickets
+    void* batch_pool = nullptr;    // batched instances: term stores, schedules, states, inputs
+    size_t batch_pool_bytes = 0;
     size_t epochs_cap = 0;
     int n_cu = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -628,6 +630,29 @@ struct Engine {
         return tk;
     }
 
+    // Kernel argument of the window of rotation steps [t, t + kw).
+    static Window<Alg, CP> make_window(const std::vector<AStep>& steps, size_t t, size_t kw, double mincoeff,
+                                       int maxdeg, int max_xw) {
+        Window<Alg, CP> w{};
+        for (size_t j = 0; j < kw; ++j) {
+            const AStep& b = steps[t + j];
+            w.g[j] = b.g;
+            w.cs[j] = make_fac<CP>(b.cs);
+            w.sn[j] = make_fac<CP>(b.sn);
+            w.thresh_mask |= b.trunc_thresh << j;
+            w.xw_mask |= b.trunc_xw << j;
+        }
+        w.k = (int)kw;
+        for (size_t j = 0; j < kw; ++j) {
+            w.lp[j] = lin_hash<NP>(w.g[j].p);
+            for (size_t i2 = 0; i2 < kw; ++i2) w.cross[j] |= (uint32_t)Alg::lin(w.g[i2].p, w.g[j]) << i2;
+        }
+        w.mincoeff = mincoeff;
+        w.maxdeg = maxdeg;
+        w.max_xw = max_xw;
+        return w;
+    }
+
     int run_async(const std::vector<AStep>& steps, const std::vector<CliffGate>& cliff, double mincoeff, int maxdeg,
                   int max_xw, uint32_t topk = 0) {
         if (steps.empty()) return FFX_OK;
@@ -697,23 +722,7 @@ struct Engine {
                 if (a.kind == 0) ctx->prof.run_launches += 1;
                 if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev], ctx->stream));
                 if (a.kind == 0 && kw > 1) {
-                    Window<Alg, CP> w{};
-                    for (size_t j = 0; j < kw; ++j) {
-                        const AStep& b = steps[t + j];
-                        w.g[j] = b.g;
-                        w.cs[j] = make_fac<CP>(b.cs);
-                        w.sn[j] = make_fac<CP>(b.sn);
-                        w.thresh_mask |= b.trunc_thresh << j;
-                        w.xw_mask |= b.trunc_xw << j;
-                    }
-                    w.k = (int)kw;
-                    for (size_t j = 0; j < kw; ++j) {
-                        w.lp[j] = lin_hash<NP>(w.g[j].p);
-                        for (size_t i2 = 0; i2 < kw; ++i2) w.cross[j] |= (uint32_t)Alg::lin(w.g[i2].p, w.g[j]) << i2;
-                    }
-                    w.mincoeff = mincoeff;
-                    w.maxdeg = maxdeg;
-                    w.max_xw = max_xw;
+                    const Window<Alg, CP> w = make_window(steps, t, kw, mincoeff, maxdeg, max_xw);
                     hipLaunchKernelGGL((k_win_async<Alg, CP, K>), dim3(win_grid), dim3(kBlockWin), 0, ctx->stream,
                                        store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, w, ep, e);
                     CHK(launch_check());
@@ -931,6 +940,44 @@ typename MajoranaAlg<W>::Gate majorana_gate_dev(const ffx_majorana_gate& g) {
     return d;
 }
 
+// Device steps of the asynchronous engine for a Pauli schedule: rotation
+// parameters, truncation points, top-k points, compaction points, Clifford blocks.
+template <int W, class CP, int K>
+void pauli_async_steps(const ffx_pauli_gate* gates, const std::vector<Step>& steps, const ffx_options& o,
+                       std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>& sd, std::vector<CliffGate>& cl) {
+    using AStep = typename Engine<PauliAlg<W>, CP, K>::AStep;
+    sd.reserve(steps.size());
+    int trunc_points = 0;
+    for (const Step& s : steps) {
+        AStep d{};
+        if (s.kind == 0) {
+            const ffx_pauli_gate& g = gates[s.gate];
+            d.g = pauli_gate_dev<W>(g);
+            gate_trig(g.theta, d.cs, d.sn);
+            d.kind = 0;
+            d.trunc_thresh = (s.trunc && o.mincoeff > 0) ? 1 : 0;
+            d.trunc_xw = (s.trunc && o.max_xweight >= 0 && o.xtrunc_period > 0 &&
+                          (s.rc + 1) % o.xtrunc_period == 0)
+                             ? 1
+                             : 0;
+            if (s.trunc && o.topk > 0) {
+                // threshold, then top-k, then the X/Y cap (truncate.h:51-57)
+                d.topk_after = 1;
+                d.topk_xw = d.trunc_xw;
+                d.trunc_xw = 0;
+            }
+            // batched runs compact at every batch end; per-gate runs every 16th truncation
+            if (s.trunc) d.compact_after = (o.batched || (++trunc_points % 16) == 0) ? 1 : 0;
+        } else {
+            d.kind = 1;
+            d.cb = (int)cl.size();
+            d.cn = (int)(s.ce - s.cb);
+            for (int64_t t = s.cb; t < s.ce; ++t) cl.push_back(CliffGate{gates[t].kind, gates[t].q0, gates[t].q1, 0});
+        }
+        sd.push_back(d);
+    }
+}
+
 template <int W, class CP, int K>
 int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const uint64_t* x, const uint64_t* z,
               const double* re, const double* im, int64_t n_terms, const ffx_options& o, ffx_terms* out,
@@ -946,36 +993,7 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
         using AStep = typename Engine<Alg, CP, K>::AStep;
         std::vector<AStep> sd;
         std::vector<CliffGate> cl;
-        sd.reserve(steps.size());
-        int trunc_points = 0;
-        for (const Step& s : steps) {
-            AStep d{};
-            if (s.kind == 0) {
-                const ffx_pauli_gate& g = gates[s.gate];
-                d.g = pauli_gate_dev<W>(g);
-                gate_trig(g.theta, d.cs, d.sn);
-                d.kind = 0;
-                d.trunc_thresh = (s.trunc && o.mincoeff > 0) ? 1 : 0;
-                d.trunc_xw = (s.trunc && o.max_xweight >= 0 && o.xtrunc_period > 0 &&
-                              (s.rc + 1) % o.xtrunc_period == 0)
-                                 ? 1
-                                 : 0;
-                if (s.trunc && o.topk > 0) {
-                    // threshold, then top-k, then the X/Y cap (truncate.h:51-57)
-                    d.topk_after = 1;
-                    d.topk_xw = d.trunc_xw;
-                    d.trunc_xw = 0;
-                }
-                // batched runs compact at every batch end; per-gate runs every 16th truncation
-                if (s.trunc) d.compact_after = (o.batched || (++trunc_points % 16) == 0) ? 1 : 0;
-            } else {
-                d.kind = 1;
-                d.cb = (int)cl.size();
-                d.cn = (int)(s.ce - s.cb);
-                for (int64_t t = s.cb; t < s.ce; ++t) cl.push_back(CliffGate{gates[t].kind, gates[t].q0, gates[t].q1, 0});
-            }
-            sd.push_back(d);
-        }
+        pauli_async_steps<W, CP, K>(gates, steps, o, sd, cl);
         CHK(eng.run_async(sd, cl, o.mincoeff, maxdeg, o.max_xweight, o.topk > 0 ? (uint32_t)o.topk : 0u));
     } else
     for (const Step& s : steps) {
@@ -1088,6 +1106,183 @@ inline int mono_degree_mod4_r(const uint64_t* m, int W) {
     return (d >> 1) & 1;
 }
 
+// Batched propagation: one workgroup per instance (k_batch_run); instances that
+// are not eligible or outgrow their slot budget run on the device-wide engine.
+template <int W, class CP, int K>
+int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, const ffx_options& o, int64_t slots,
+                    double* out_expect, int64_t* out_terms, int32_t* out_fallback) {
+    using Alg = PauliAlg<W>;
+    using Eng = Engine<Alg, CP, K>;
+    using AStep = typename Eng::AStep;
+    using store_t = typename CP::store_t;
+    constexpr int NP = Alg::NP;
+    const double t0 = now_s();
+    const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
+    int64_t max_terms = 1;
+    for (int32_t i = 0; i < n_inst; ++i) max_terms = std::max(max_terms, inst[i].n_terms);
+    const size_t S = pow2ceil((size_t)std::max<int64_t>(std::max<int64_t>(slots > 0 ? slots : 32768, 2 * max_terms), 1024));
+    if (S > 0x40000000ull) return set_err(ctx, FFX_ECAPACITY, "slot budget too large");
+    // schedules: eligible instances get their windows, the others the device-wide engine
+    std::vector<int32_t> elig;
+    std::vector<std::vector<BStep<Alg, CP>>> sched(n_inst);
+    for (int32_t i = 0; i < n_inst; ++i) {
+        bool ok = o.topk <= 0 && (size_t)inst[i].n_terms <= S / 2;
+        for (int64_t g = 0; g < inst[i].n_gates && ok; ++g) ok = inst[i].gates[g].kind == FFX_ROT;
+        if (!ok) continue;
+        const std::vector<Step> steps = pauli_schedule<W>(inst[i].gates, inst[i].n_gates, o.batched != 0);
+        std::vector<AStep> sd;
+        std::vector<CliffGate> cl;
+        pauli_async_steps<W, CP, K>(inst[i].gates, steps, o, sd, cl);
+        for (size_t t = 0; t < sd.size();) {
+            const size_t kw = Eng::window_len(sd, t, kWin);
+            BStep<Alg, CP> b{};
+            b.w = Eng::make_window(sd, t, kw, o.mincoeff, maxdeg, o.max_xweight);
+            sched[i].push_back(b);
+            t += kw;
+        }
+        elig.push_back(i);
+    }
+    const size_t ne = elig.size();
+    int32_t fallback = n_inst - (int32_t)ne;
+    if (ne > 0) {
+        // one pool: per-instance stores, then the states, schedules and inputs
+        const size_t hs = 2 * S;
+        const size_t store_bytes = (2 * NP * S) * 8 + 2 * (size_t)K * CP::NS * S * sizeof(store_t) + hs * 8;
+        size_t n_steps = 0, n_in = 0;
+        for (int32_t i : elig) {
+            n_steps += sched[i].size();
+            n_in += (size_t)inst[i].n_terms;
+        }
+        auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+        const size_t off_states = al(ne * store_bytes);
+        const size_t off_steps = off_states + al(ne * sizeof(InstState<CP>));
+        const size_t off_x = off_steps + al(std::max<size_t>(n_steps, 1) * sizeof(BStep<Alg, CP>));
+        const size_t off_z = off_x + al(std::max<size_t>(n_in, 1) * W * 8);
+        const size_t off_re = off_z + al(std::max<size_t>(n_in, 1) * W * 8);
+        const size_t off_im = off_re + al(std::max<size_t>(n_in, 1) * 8);
+        const size_t off_off = off_im + al(std::max<size_t>(n_in, 1) * 8);
+        const size_t total = off_off + al((ne + 1) * 8);
+        if (total > ctx->batch_pool_bytes) {
+            dev_free(ctx->batch_pool);
+            ctx->batch_pool_bytes = 0;
+            CHK(dev_alloc(ctx, &ctx->batch_pool, total));
+            ctx->batch_pool_bytes = total;
+        }
+        char* pool = (char*)ctx->batch_pool;
+        std::vector<InstState<CP>> st(ne);
+        std::vector<BStep<Alg, CP>> hsteps;
+        hsteps.reserve(n_steps);
+        std::vector<uint64_t> hx, hz;
+        std::vector<double> hre, him;
+        std::vector<int64_t> hoff(ne + 1, 0);
+        hx.reserve(n_in * W);
+        hz.reserve(n_in * W);
+        for (size_t b = 0; b < ne; ++b) {
+            const ffx_instance& in = inst[elig[b]];
+            char* base = pool + b * store_bytes;
+            InstState<CP>& s = st[b];
+            s = InstState<CP>{};
+            s.key = (uint64_t*)base;
+            s.key2 = s.key + NP * S;
+            s.coef = (store_t*)(s.key2 + NP * S);
+            s.coef2 = s.coef + (size_t)K * CP::NS * S;
+            s.index = (uint64_t*)(s.coef2 + (size_t)K * CP::NS * S);
+            s.hmask = hs - 1;
+            s.cap = (uint32_t)S;
+            s.n = (uint32_t)in.n_terms;
+            s.steps = (const void*)(pool + off_steps + hsteps.size() * sizeof(BStep<Alg, CP>));
+            s.n_steps = (uint32_t)sched[elig[b]].size();
+            for (int w = 0; w < FFX_MAX_WORDS_DEV; ++w) s.state[w] = in.basis_state[w];
+            hsteps.insert(hsteps.end(), sched[elig[b]].begin(), sched[elig[b]].end());
+            for (int64_t t = 0; t < in.n_terms; ++t) {
+                for (int w = 0; w < W; ++w) {
+                    hx.push_back(in.x[t * W + w]);
+                    hz.push_back(in.z[t * W + w]);
+                }
+                hre.push_back(in.c_re[t]);
+                him.push_back(in.c_im ? in.c_im[t] : 0.0);
+            }
+            hoff[b + 1] = hoff[b] + in.n_terms;
+        }
+        InstState<CP>* dst = (InstState<CP>*)(pool + off_states);
+        HIPCHK(hipMemcpyAsync(dst, st.data(), ne * sizeof(InstState<CP>), hipMemcpyHostToDevice, ctx->stream));
+        if (!hsteps.empty())
+            HIPCHK(hipMemcpyAsync(pool + off_steps, hsteps.data(), hsteps.size() * sizeof(BStep<Alg, CP>),
+                                  hipMemcpyHostToDevice, ctx->stream));
+        if (n_in) {
+            HIPCHK(hipMemcpyAsync(pool + off_x, hx.data(), hx.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(pool + off_z, hz.data(), hz.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(pool + off_re, hre.data(), hre.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(pool + off_im, him.data(), him.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        }
+        HIPCHK(hipMemcpyAsync(pool + off_off, hoff.data(), hoff.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        hipLaunchKernelGGL((k_batch_unpack<W, CP, K>), dim3((unsigned)ne), dim3(kBlock), 0, ctx->stream, dst,
+                           (const uint64_t*)(pool + off_x), (const uint64_t*)(pool + off_z),
+                           (const double*)(pool + off_re), (const double*)(pool + off_im),
+                           (const int64_t*)(pool + off_off));
+        HIPCHK(hipGetLastError());
+        if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
+        hipLaunchKernelGGL((k_batch_run<Alg, CP, K>), dim3((unsigned)ne), dim3(kBlockBatch), 0, ctx->stream, dst);
+        HIPCHK(hipGetLastError());
+        if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
+        ctx->launches += 2;
+        HIPCHK(hipMemcpyAsync(st.data(), dst, ne * sizeof(InstState<CP>), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->kernel_timing) {
+            float ms = 0.f;
+            HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
+            ctx->prof.seconds_kernel += 1e-3 * ms;
+            ctx->prof.timed_launches += 1;
+        }
+        ctx->prof.run_launches += 1;
+        for (size_t b = 0; b < ne; ++b) {
+            const int32_t i = elig[b];
+            if (st[b].status != 0) {
+                if (st[b].status == 2) return set_err(ctx, FFX_EHIP, "internal error: hash index full");
+                sched[i].clear();
+                continue;
+            }
+            out_expect[2 * i] = st[b].expect[0];
+            out_expect[2 * i + 1] = st[b].expect[1];
+            if (out_terms) out_terms[i] = (int64_t)st[b].n - (int64_t)st[b].dead;
+            for (const auto& bs : sched[i]) ctx->prof.n_gates += bs.w.k;
+            ctx->prof.updates += (int64_t)st[b].updates;
+            ctx->prof.outs += (int64_t)st[b].outs;
+            ctx->prof.peak_terms = std::max<int64_t>(ctx->prof.peak_terms, (int64_t)st[b].peak);
+            sched[i].push_back(BStep<Alg, CP>{});  // marks "done"
+        }
+    }
+    // the rest on the device-wide engine, one instance at a time
+    ffx_profile acc = ctx->prof;
+    for (int32_t i = 0; i < n_inst; ++i) {
+        if (!sched[i].empty()) continue;
+        if (std::find(elig.begin(), elig.end(), i) != elig.end()) ++fallback;
+        const ffx_instance& in = inst[i];
+        double ex[2] = {0.0, 0.0};
+        ctx->prof = ffx_profile{};
+        CHK((run_pauli<W, CP, K>(ctx, in.gates, in.n_gates, in.x, in.z, in.c_re, K == 2 ? in.c_im : nullptr,
+                                 in.n_terms, o, nullptr, in.basis_state, ex)));
+        out_expect[2 * i] = ex[0];
+        out_expect[2 * i + 1] = ex[1];
+        if (out_terms) out_terms[i] = (int64_t)ctx->ctrl_host->n - (int64_t)ctx->ctrl_host->n_dead;
+        acc.n_gates += ctx->prof.n_gates;
+        acc.updates += ctx->prof.updates;
+        acc.outs += ctx->prof.outs;
+        acc.peak_terms = std::max(acc.peak_terms, ctx->prof.peak_terms);
+        acc.compactions += ctx->prof.compactions;
+        acc.run_launches += ctx->prof.run_launches;
+        acc.seconds_kernel += ctx->prof.seconds_kernel;
+        acc.timed_launches += ctx->prof.timed_launches;
+        acc.seconds_enqueue += ctx->prof.seconds_enqueue;
+    }
+    ctx->prof = acc;
+    ctx->prof.seconds_total = now_s() - t0;
+    ctx->prof.seconds_propagate = ctx->prof.seconds_total;
+    ctx->last_kind = 0;  // the batch leaves no single device-resident result
+    if (out_fallback) *out_fallback = fallback;
+    return FFX_OK;
+}
+
 // Precision and coefficient-width dispatch for one key width W.  Each W is
 // instantiated in its own translation unit.
 template <int W>
@@ -1125,6 +1320,19 @@ int dispatch_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_ga
 }
 
 template <int W>
+int dispatch_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, const ffx_options& o, int64_t slots,
+                   double* out_expect, int64_t* out_terms, int32_t* out_fallback, int prec, int K) {
+    if (prec == FFX_FP64)
+        return K == 1 ? run_pauli_batch<W, F64, 1>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback)
+                      : run_pauli_batch<W, F64, 2>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback);
+    if (prec == FFX_FP32)
+        return K == 1 ? run_pauli_batch<W, F32, 1>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback)
+                      : run_pauli_batch<W, F32, 2>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback);
+    return K == 1 ? run_pauli_batch<W, F32C, 1>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback)
+                  : run_pauli_batch<W, F32C, 2>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback);
+}
+
+template <int W>
 int dispatch_expect(ffx_ctx* ctx, const uint64_t* state, double* out, int kind, int prec, int K) {
     if (kind == 1) {
         using A = PauliAlg<W>;
@@ -1145,6 +1353,8 @@ int dispatch_expect(ffx_ctx* ctx, const uint64_t* state, double* out, int kind, 
     EXT template int dispatch_majorana<W_>(ffx_ctx*, const ffx_majorana_gate*, int64_t, const uint64_t*,           \
                                            const double*, const double*, int64_t, int, double, ffx_terms*,         \
                                            const uint64_t*, double*, int, int);                                    \
-    EXT template int dispatch_expect<W_>(ffx_ctx*, const uint64_t*, double*, int, int, int)
+    EXT template int dispatch_expect<W_>(ffx_ctx*, const uint64_t*, double*, int, int, int);                      \
+    EXT template int dispatch_batch<W_>(ffx_ctx*, const ffx_instance*, int32_t, const ffx_options&, int64_t,       \
+                                        double*, int64_t*, int32_t*, int, int)
 
 }  // namespace ffx_host

@@ -67,6 +67,7 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     dev_free(ctx->cliff);
     dev_free(ctx->epochs);
     dev_free(ctx->topk_buf);
+    dev_free(ctx->batch_pool);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -145,6 +146,53 @@ int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gat
         case 2: return dispatch_pauli<2>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
         case 3: return dispatch_pauli<3>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
         default: return dispatch_pauli<4>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
+    }
+}
+
+int ffx_pauli_propagate_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
+                              const ffx_options* opts, int64_t slots, double* out_expect, int64_t* out_terms,
+                              int32_t* out_fallback) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    if (!opts || n_inst < 0 || (n_inst > 0 && (!inst || !out_expect)) || slots < 0)
+        return set_err(ctx, FFX_EINVAL, "invalid arguments");
+    if (n_words < 1 || n_words > FFX_MAX_WORDS) return set_err(ctx, FFX_EINVAL, "n_words must be in 1..4");
+    int K = 1;
+    for (int32_t i = 0; i < n_inst; ++i) {
+        const ffx_instance& in = inst[i];
+        if (in.n_gates < 0 || in.n_terms < 0 || (in.n_gates > 0 && !in.gates) ||
+            (in.n_terms > 0 && (!in.x || !in.z || !in.c_re)))
+            return set_err(ctx, FFX_EINVAL, "invalid instance " + std::to_string(i));
+        if (in.n_terms > 0x3FFFFFFF) return set_err(ctx, FFX_ECAPACITY, "too many input terms");
+        for (int64_t g = 0; g < in.n_gates; ++g) {
+            const ffx_pauli_gate& gt = in.gates[g];
+            if (gt.kind < FFX_ROT || gt.kind > FFX_CZ) return set_err(ctx, FFX_EINVAL, "unknown gate kind");
+            if (gt.kind == FFX_ROT) {
+                for (int w = n_words; w < FFX_MAX_WORDS; ++w)
+                    if (gt.x[w] || gt.z[w]) return set_err(ctx, FFX_EINVAL, "rotation axis exceeds n_words");
+            } else {
+                const int lim = 64 * n_words;
+                const bool two = gt.kind >= FFX_CNOT;
+                if (gt.q0 < 0 || gt.q0 >= lim || (two && (gt.q1 < 0 || gt.q1 >= lim || gt.q1 == gt.q0)))
+                    return set_err(ctx, FFX_EINVAL, "Clifford qubit out of range");
+            }
+        }
+        if (in.c_im)
+            for (int64_t t = 0; t < in.n_terms && K == 1; ++t)
+                if (in.c_im[t] != 0.0) K = 2;
+    }
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, FFX_ENODEV, "hipSetDevice failed");
+    ctx->prof = ffx_profile{};
+    ctx->launches = 0;
+    const int prec = opts->precision;
+    if (prec < FFX_FP64 || prec > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
+    if (out_fallback) *out_fallback = 0;
+    if (n_inst == 0) return FFX_OK;
+    switch (n_words) {
+        case 1: return dispatch_batch<1>(ctx, inst, n_inst, *opts, slots, out_expect, out_terms, out_fallback, prec, K);
+        case 2: return dispatch_batch<2>(ctx, inst, n_inst, *opts, slots, out_expect, out_terms, out_fallback, prec, K);
+        case 3: return dispatch_batch<3>(ctx, inst, n_inst, *opts, slots, out_expect, out_terms, out_fallback, prec, K);
+        default: return dispatch_batch<4>(ctx, inst, n_inst, *opts, slots, out_expect, out_terms, out_fallback, prec, K);
     }
 }
 

@@ -290,11 +290,30 @@ __device__ __forceinline__ void put_value(typename CP::store_t* cout, size_t cap
     }
 }
 
+template <class Alg>
+__device__ __forceinline__ uint32_t subsets_of(uint32_t amask) {
+    uint32_t valid = 0;
+    sfor<kOrbit>([&](auto TT) {
+        constexpr int T = decltype(TT)::value;
+        if ((T & ~amask) == 0) valid |= 1u << T;
+    });
+    return valid;
+}
+
+// Where appends are counted and overflow is reported: the epoch arrays of the
+// device-wide engine, or LDS counters of a one-block instance.
+struct AppendSink {
+    uint32_t* ctr;         // next free slot offset (added to n)
+    uint32_t* flag;        // set to 1 when an append does not fit
+    uint32_t* flag_epoch;  // receives `e` on overflow (may be null)
+    uint32_t e;
+};
+
 // Appends the fresh orbit members (absent, live at the end) of the leaders of
 // this block iteration: one atomic per block and iteration.
 template <class Alg, class CP, int K>
 __device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP::store_t* cout,
-                                             const Window<Alg, CP>& w, const Epochs& ep, uint32_t e, uint32_t n,
+                                             const Window<Alg, CP>& w, const AppendSink& sink, uint32_t n,
                                              const uint64_t (&k)[Alg::NP], uint64_t lk,
                                              const typename CP::val_t (&v)[kOrbit][K], uint32_t fresh,
                                              uint32_t* sbase) {
@@ -303,7 +322,7 @@ __device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP
     uint32_t tot;
     const uint32_t rank = block_exclusive_scan(napp, &tot);
     if (tot) {
-        if (threadIdx.x == 0) *sbase = atomicAdd(&ep.app[e], tot);
+        if (threadIdx.x == 0) *sbase = atomicAdd(sink.ctr, tot);
         __syncthreads();
         if (napp) {
             uint64_t at = (uint64_t)n + *sbase + rank;
@@ -316,8 +335,8 @@ __device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP
                             orbit_key<T>(k, w, kt);
                             put_slot_h<NP, CP, K>(st, st.key, cout, at, kt, v[T], hmix(lk ^ lp_of<T>(w)));
                         } else {
-                            atomicOr(&ep.flags[0], 1u);
-                            ep.flags[1] = e;
+                            atomicOr(sink.flag, 1u);
+                            if (sink.flag_epoch) *sink.flag_epoch = sink.e;
                         }
                         ++at;
                     }
@@ -325,6 +344,54 @@ __device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP
             });
         }
     }
+}
+
+// One tile of blockDim slots [b0, b0 + blockDim) of a window: every slot in
+// [0, n) replays its orbit and writes its own value; the leaders append.
+// Block-uniform (contains barriers).
+template <class Alg, class CP, int K>
+__device__ __forceinline__ void win_tile(const Store<CP, K>& st, const typename CP::store_t* cin,
+                                         typename CP::store_t* cout, const Window<Alg, CP>& w, uint32_t n,
+                                         uint32_t b0, const AppendSink& sink, uint32_t* sbase, int& dd,
+                                         int (&sdl)[kWin]) {
+    constexpr int NP = Alg::NP;
+    using val_t = typename CP::val_t;
+    const size_t cap = st.cap;
+    const uint32_t i = b0 + threadIdx.x;
+    uint32_t fresh = 0;
+    uint64_t k[NP];
+    uint64_t lk = 0;
+    val_t v[kOrbit][K];
+    if (i < n) {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) v[0][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, i);
+        const uint32_t amask = window_amask<Alg>(k, w.g, w.k);
+        const uint32_t valid = subsets_of<Alg>(amask);
+        uint32_t alive = CP::dead(v[0][0]) ? 0u : 1u, exist = 1;
+        uint32_t slot[kOrbit];
+        OrbitFacts f{0u, 0u, 0u};
+        bool leader = true;
+        if (amask) {
+            lk = lin_hash<NP>(k);
+            f = self_facts<Alg, CP>(k, w, amask);
+            orbit_gather<Alg, CP, K>(st, cin, w, k, lk, valid, n, slot, v, exist, alive, f);
+            uint32_t lead = i;
+            sfor<kOrbit>([&](auto TT) {
+                constexpr int T = decltype(TT)::value;
+                if constexpr (T > 0)
+                    if ((exist >> T) & 1) lead = min(lead, slot[T]);
+            });
+            leader = lead == i;
+        }
+        const uint32_t a0 = alive & 1u;
+        orbit_replay<Alg, CP, K>(w, k, amask, valid, f, v, alive, sdl, leader);
+        put_value<CP, K>(cout, cap, i, alive & 1u, v[0]);
+        dd += (int)a0 - (int)(alive & 1u);
+        if (leader) fresh = alive & ~exist;
+    }
+    orbit_append<Alg, CP, K>(st, cout, w, sink, n, k, lk, v, fresh, sbase);
 }
 
 __device__ __forceinline__ void window_report(const Epochs& ep, uint32_t e, int dd, const int (&sdl)[kWin]) {
@@ -338,69 +405,22 @@ __device__ __forceinline__ void window_report(const Epochs& ep, uint32_t e, int 
     }
 }
 
-template <class Alg>
-__device__ __forceinline__ uint32_t subsets_of(uint32_t amask) {
-    uint32_t valid = 0;
-    sfor<kOrbit>([&](auto TT) {
-        constexpr int T = decltype(TT)::value;
-        if ((T & ~amask) == 0) valid |= 1u << T;
-    });
-    return valid;
-}
 
-// Every slot replays its own orbit.
+// Every slot replays its own orbit (device-wide engine, one window per launch).
 template <class Alg, class CP, int K>
 __global__ void __launch_bounds__(kBlockWin) k_win_async(Store<CP, K> st, const typename CP::store_t* cin,
                                                          typename CP::store_t* cout, Window<Alg, CP> w, Epochs ep,
                                                          uint32_t e) {
-    constexpr int NP = Alg::NP;
-    using val_t = typename CP::val_t;
     uint32_t n, dead;
     if (!step_enter(ep, e, n, dead)) return;
-    const size_t cap = st.cap;
     // block-strided tiles of blockDim slots
-    const uint32_t begin = blockIdx.x * blockDim.x, end = n, step = gridDim.x * blockDim.x;
-    if (begin >= end) return;  // block-uniform: nothing to do, nothing to report
+    const uint32_t begin = blockIdx.x * blockDim.x, step = gridDim.x * blockDim.x;
+    if (begin >= n) return;  // block-uniform: nothing to do, nothing to report
     __shared__ uint32_t sbase;
+    const AppendSink sink{&ep.app[e], &ep.flags[0], &ep.flags[1], e};
     int dd = 0;
     int sdl[kWin] = {0, 0, 0, 0};
-    for (uint32_t b0 = begin; b0 < end; b0 += step) {
-        const uint32_t i = b0 + threadIdx.x;
-        uint32_t fresh = 0;
-        uint64_t k[NP];
-        uint64_t lk = 0;
-        val_t v[kOrbit][K];
-        if (i < end) {
-#pragma unroll
-            for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
-#pragma unroll
-            for (int kk = 0; kk < K; ++kk) v[0][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, i);
-            const uint32_t amask = window_amask<Alg>(k, w.g, w.k);
-            const uint32_t valid = subsets_of<Alg>(amask);
-            uint32_t alive = CP::dead(v[0][0]) ? 0u : 1u, exist = 1;
-            uint32_t slot[kOrbit];
-            OrbitFacts f{0u, 0u, 0u};
-            bool leader = true;
-            if (amask) {
-                lk = lin_hash<NP>(k);
-                f = self_facts<Alg, CP>(k, w, amask);
-                orbit_gather<Alg, CP, K>(st, cin, w, k, lk, valid, n, slot, v, exist, alive, f);
-                uint32_t lead = i;
-                sfor<kOrbit>([&](auto TT) {
-                    constexpr int T = decltype(TT)::value;
-                    if constexpr (T > 0)
-                        if ((exist >> T) & 1) lead = min(lead, slot[T]);
-                });
-                leader = lead == i;
-            }
-            const uint32_t a0 = alive & 1u;
-            orbit_replay<Alg, CP, K>(w, k, amask, valid, f, v, alive, sdl, leader);
-            put_value<CP, K>(cout, cap, i, alive & 1u, v[0]);
-            dd += (int)a0 - (int)(alive & 1u);
-            if (leader) fresh = alive & ~exist;
-        }
-        orbit_append<Alg, CP, K>(st, cout, w, ep, e, n, k, lk, v, fresh, &sbase);
-    }
+    for (uint32_t b0 = begin; b0 < n; b0 += step) win_tile<Alg, CP, K>(st, cin, cout, w, n, b0, sink, &sbase, dd, sdl);
     window_report(ep, e, dd, sdl);
 }
 

@@ -119,3 +119,29 @@ def read_circuit(path: str):
     if len(recs) != n:
         raise ValueError(f"{path}: header says {n} gates, found {len(recs)}")
     return np.concatenate(recs) if recs else np.zeros(0, PAULI_GATE_DTYPE)
+
+
+def batch_kicked_ising(n_inst: int = 512, seed: int = 4242, l_lo: int = 6, l_span: int = 7, d_lo: int = 4,
+                       d_span: int = 21, zz: float = -math.pi / 2):
+    """Uneven kicked-Ising instances (the batched configuration): instance i has
+    an L x L lattice with L = l_lo + (u mod l_span), depth d_lo + (u mod d_span)
+    layers, per-qubit RX angles uniform in [0, pi/2], and a random weight-1 or
+    weight-2 Z observable, all drawn from one SplitMix64 stream.  Returns a list
+    of (gates, x, z, c, L, depth) with keys packed on words_for_bits(L*L) words."""
+    rng = SplitMix64(seed)
+    out = []
+    for _ in range(n_inst):
+        L = l_lo + int(rng.next() % l_span)
+        depth = d_lo + int(rng.next() % d_span)
+        nq = L * L
+        thetas = [0.5 * math.pi * rng.uniform() for _ in range(nq)]
+        gates, _ = kicked_ising(L, depth, zz=zz, thetas=thetas)
+        w = 1 + int(rng.next() % 2)
+        sites = []
+        while len(sites) < w:
+            q = int(rng.next() % nq)
+            if q not in sites:
+                sites.append(q)
+        x, z, c = single_pauli("Z" * w, sorted(sites), nq)
+        out.append((gates, x, z, c, L, depth))
+    return out

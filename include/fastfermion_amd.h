@@ -149,6 +149,32 @@ int ffx_majorana_propagate(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t
                            int32_t maxdegree, double mincoeff, int32_t precision, ffx_terms* out,
                            const uint64_t* occ, double out_expect[2]);
 
+/* One circuit instance of a batched Pauli propagation. */
+typedef struct ffx_instance {
+    const ffx_pauli_gate* gates; /* circuit, gate order as in ffx_pauli_propagate */
+    int64_t n_gates;
+    const uint64_t* x;           /* observable: n_terms x n_words masks, term-major */
+    const uint64_t* z;
+    const double* c_re;
+    const double* c_im;          /* NULL: real coefficients */
+    int64_t n_terms;
+    uint64_t basis_state[4];     /* readout state |b> (bit q of word q/64) */
+} ffx_instance;
+
+/* Batched Pauli propagation (no reference counterpart: the reference runs one
+ * circuit per propagate call, src/pauli/propagate.h:297-493, and a batch is a
+ * loop over such calls).  Every instance runs the reference schedule and
+ * truncation of ffx_pauli_propagate with the shared options; all instances
+ * run in one launch, one workgroup each, with `slots` term slots per instance
+ * (0: a default).  Instances whose terms outgrow that budget, that contain
+ * Clifford gates or that ask for top-k are run afterwards on the device-wide
+ * engine, so every result is complete.  out_expect[2i], [2i+1] receives
+ * <b_i|O_i|b_i>; out_terms[i] (optional) the live term count at the end;
+ * out_fallback (optional) how many instances took the device-wide engine. */
+int ffx_pauli_propagate_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
+                              const ffx_options* opts, int64_t slots, double* out_expect, int64_t* out_terms,
+                              int32_t* out_fallback);
+
 /* Expectation of the device-resident result of the last propagate call on a
  * product state (Pauli: qubit bits b; Majorana: mode occupations).
  * Replaces QubitProductState::operator() (src/qubitproductstate.h:82-88),

@@ -1,0 +1,145 @@
+"""Batched propagation (one workgroup per circuit instance, ffx_pauli_propagate_batch)
+against the oracle run instance by instance, and against the device-wide
+engine.  The batch runs each instance's reference schedule and truncation, so
+FP64 expectations agree with the oracle to rounding of the final sum and the
+live term counts and work counters are identical; instances that outgrow their
+slot budget, contain Clifford gates or ask for top-k must come back complete
+through the device-wide engine."""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import pyffo
+from fastfermion_amd import _abi, workloads as wl
+from fastfermion_amd._native import Context
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def ctx():
+    c = Context(0)
+    yield c
+    c.close()
+
+
+def pad(a, W):
+    a = np.asarray(a, np.uint64)
+    out = np.zeros((a.shape[0], W), np.uint64)
+    out[:, : a.shape[1]] = a
+    return out
+
+
+def instances(n, seed, **kw):
+    return wl.batch_kicked_ising(n, seed=seed, **kw)
+
+
+def run_both(ctx, inst, W, states=None, **kw):
+    items = []
+    refs = []
+    for i, (g, x, z, c, L, d) in enumerate(inst):
+        st = None if states is None else states[i]
+        items.append((g, pad(x, W), pad(z, W), c, st))
+        r = pyffo.pauli_propagate(g, W, pad(x, W), pad(z, W), c,
+                                  **{k: v for k, v in kw.items() if k not in ("precision", "slots")})
+        b = [0] * W
+        if st is not None:
+            b[: len(st)] = st
+        refs.append((pyffo.pauli_expect(r, b), len(r.c), r.updates))
+    ex, nt, fb = ctx.pauli_propagate_batch(items, W, **kw)
+    return ex, nt, fb, refs
+
+
+@pytest.mark.parametrize("batched", [False, True])
+def test_batch_matches_oracle_per_instance(ctx, batched):
+    inst = instances(24, 11, l_lo=3, l_span=4, d_lo=2, d_span=5)
+    W = 1
+    kw = dict(maxdegree=8, mincoeff=1e-4, batched=batched)
+    # batch-end truncation lets terms pile up inside each commuting batch (the
+    # RZZ(-pi/2) batches leave tiny copies alive): give those a larger budget;
+    # instances that still outgrow it are exact through the fallback
+    ex, nt, fb, refs = run_both(ctx, inst, W, slots=(1 << 17) if batched else 0, **kw)
+    if not batched:
+        assert fb == 0
+    upd = 0
+    for i, (e, n, u) in enumerate(refs):
+        assert abs(ex[i] - e) < 1e-12, (i, ex[i], e)
+        assert nt[i] == n, (i, nt[i], n)
+        upd += u
+    assert ctx.profile()["updates"] == upd
+
+
+def test_batch_uneven_two_words_and_states(ctx):
+    inst = instances(16, 5, l_lo=6, l_span=5, d_lo=3, d_span=6)
+    W = 2
+    rng = np.random.default_rng(3)
+    states = [[int(rng.integers(0, 2 ** 63)), int(rng.integers(0, 2 ** 36))] for _ in inst]
+    ex, nt, fb, refs = run_both(ctx, inst, W, states=states, maxdegree=8, mincoeff=1e-4, batched=False)
+    for i, (e, n, u) in enumerate(refs):
+        assert abs(ex[i] - e) < 1e-12
+        assert nt[i] == n
+
+
+def test_batch_slot_budget_fallback(ctx):
+    """A budget far below the instances' peaks: they stop in the batch launch
+    and rerun on the device-wide engine with identical results."""
+    inst = instances(6, 17, l_lo=7, l_span=2, d_lo=8, d_span=3)
+    W = 1
+    kw = dict(maxdegree=8, mincoeff=1e-5, batched=True)
+    ex, nt, fb, refs = run_both(ctx, inst, W, slots=1024, **kw)
+    assert fb > 0
+    for i, (e, n, u) in enumerate(refs):
+        assert abs(ex[i] - e) < 1e-12
+        assert nt[i] == n
+
+
+def test_batch_ineligible_instances(ctx):
+    """Clifford gates and top-k go to the device-wide engine; mixed with normal
+    instances in one call."""
+    inst = instances(4, 23, l_lo=3, l_span=2, d_lo=2, d_span=3)
+    W = 1
+    items, refs = [], []
+    for j, (g, x, z, c, L, d) in enumerate(inst):
+        if j == 1:
+            g = np.concatenate([g[:5], _abi.clifford_record(_abi.H, 0, 0), g[5:]])
+        items.append((g, x, z, c, None))
+        r = pyffo.pauli_propagate(g, W, x, z, c, maxdegree=8, mincoeff=1e-4)
+        refs.append(pyffo.pauli_expect(r, [0]))
+    ex, nt, fb = ctx.pauli_propagate_batch(items, W, maxdegree=8, mincoeff=1e-4)
+    assert fb >= 1  # the Clifford instance, plus any that outgrew the default budget
+    for i, e in enumerate(refs):
+        assert abs(ex[i] - e) < 1e-12
+    ex2, _, fb2 = ctx.pauli_propagate_batch(items, W, maxdegree=8, mincoeff=1e-4, topk=50)
+    assert fb2 == len(items)
+
+
+@pytest.mark.parametrize("prec,tol", [(_abi.FP32C, 1e-9), (_abi.FP32, 1e-4)])
+def test_batch_precisions(ctx, prec, tol):
+    inst = instances(12, 29, l_lo=4, l_span=4, d_lo=3, d_span=6)
+    W = 1
+    items = [(g, x, z, c, None) for (g, x, z, c, L, d) in inst]
+    kw = dict(maxdegree=8, mincoeff=1e-4, batched=False)
+    e64, _, _ = ctx.pauli_propagate_batch(items, W, **kw)
+    e32, _, _ = ctx.pauli_propagate_batch(items, W, precision=prec, **kw)
+    assert np.max(np.abs(e64 - e32)) < tol
+
+
+def test_batch_equals_single_engine(ctx):
+    inst = instances(5, 31, l_lo=5, l_span=3, d_lo=4, d_span=4)
+    W = 1
+    items = [(g, x, z, c, None) for (g, x, z, c, L, d) in inst]
+    kw = dict(maxdegree=8, mincoeff=1e-4, batched=True)
+    ex, nt, _ = ctx.pauli_propagate_batch(items, W, **kw)
+    for i, (g, x, z, c, _) in enumerate(items):
+        res, e = ctx.pauli_propagate(g, W, x, z, c, state=[0], **kw)
+        assert abs(ex[i] - e) < 1e-12
+        assert nt[i] == len(res)
+
+
+def test_batch_empty_and_trivial(ctx):
+    ex, nt, fb = ctx.pauli_propagate_batch([], 1)
+    assert len(ex) == 0 and fb == 0
+    x, z, c = wl.single_pauli("Z", [0], 4)
+    ex, nt, fb = ctx.pauli_propagate_batch([(np.zeros(0, _abi.PAULI_GATE_DTYPE), x, z, c, None)], 1)
+    assert ex[0] == 1.0 and nt[0] == 1
