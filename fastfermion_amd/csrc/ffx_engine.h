@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <atomic>
 #include <chrono>
 #include <cmath>
 #include <cstdio>
@@ -20,6 +21,7 @@
 #include <string>
 #include <type_traits>
 #include <utility>
+#include <thread>
 #include <vector>
 
 #include "../../include/fastfermion_amd.h"
@@ -1123,12 +1125,12 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     const size_t S = pow2ceil((size_t)std::max<int64_t>(std::max<int64_t>(slots > 0 ? slots : 32768, 2 * max_terms), 1024));
     if (S > 0x40000000ull) return set_err(ctx, FFX_ECAPACITY, "slot budget too large");
     // schedules: eligible instances get their windows, the others the device-wide engine
-    std::vector<int32_t> elig;
     std::vector<std::vector<BStep<Alg, CP>>> sched(n_inst);
-    for (int32_t i = 0; i < n_inst; ++i) {
-        bool ok = o.topk <= 0 && (size_t)inst[i].n_terms <= S / 2;
-        for (int64_t g = 0; g < inst[i].n_gates && ok; ++g) ok = inst[i].gates[g].kind == FFX_ROT;
-        if (!ok) continue;
+    std::vector<char> ok(n_inst, 0), done(n_inst, 0);
+    auto build = [&](int32_t i) {
+        bool good = o.topk <= 0 && (size_t)inst[i].n_terms <= S / 2;
+        for (int64_t g = 0; g < inst[i].n_gates && good; ++g) good = inst[i].gates[g].kind == FFX_ROT;
+        if (!good) return;
         const std::vector<Step> steps = pauli_schedule<W>(inst[i].gates, inst[i].n_gates, o.batched != 0);
         std::vector<AStep> sd;
         std::vector<CliffGate> cl;
@@ -1140,8 +1142,25 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             sched[i].push_back(b);
             t += kw;
         }
-        elig.push_back(i);
+        ok[i] = 1;
+    };
+    {
+        // schedules of the instances are independent: build them on host threads
+        const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<int32_t> next{0};
+        std::vector<std::thread> pool;
+        for (int t = 0; t < std::min<int32_t>(nth, n_inst); ++t)
+            pool.emplace_back([&]() {
+                for (int32_t i = next++; i < n_inst; i = next++) build(i);
+            });
+        for (auto& th : pool) th.join();
     }
+    // longest schedules first: workgroups are dispatched in launch order
+    std::vector<int32_t> elig;
+    for (int32_t i = 0; i < n_inst; ++i)
+        if (ok[i]) elig.push_back(i);
+    std::stable_sort(elig.begin(), elig.end(),
+                     [&](int32_t a, int32_t b) { return sched[a].size() > sched[b].size(); });
     const size_t ne = elig.size();
     int32_t fallback = n_inst - (int32_t)ne;
     if (ne > 0) {
@@ -1222,7 +1241,12 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
                            (const int64_t*)(pool + off_off));
         HIPCHK(hipGetLastError());
         if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        hipLaunchKernelGGL((k_batch_run<Alg, CP, K>), dim3((unsigned)ne), dim3(kBlockBatch), 0, ctx->stream, dst);
+        unsigned bb = kBlockBatch;
+        if (const char* ev = getenv("FFX_BATCH_BLOCK")) {
+            const int v = atoi(ev);
+            if (v == 64 || v == 128 || v == 256 || v == 512) bb = (unsigned)v;
+        }
+        hipLaunchKernelGGL((k_batch_run<Alg, CP, K>), dim3((unsigned)ne), dim3(bb), 0, ctx->stream, dst);
         HIPCHK(hipGetLastError());
         if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         ctx->launches += 2;
@@ -1239,7 +1263,6 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             const int32_t i = elig[b];
             if (st[b].status != 0) {
                 if (st[b].status == 2) return set_err(ctx, FFX_EHIP, "internal error: hash index full");
-                sched[i].clear();
                 continue;
             }
             out_expect[2 * i] = st[b].expect[0];
@@ -1249,14 +1272,14 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             ctx->prof.updates += (int64_t)st[b].updates;
             ctx->prof.outs += (int64_t)st[b].outs;
             ctx->prof.peak_terms = std::max<int64_t>(ctx->prof.peak_terms, (int64_t)st[b].peak);
-            sched[i].push_back(BStep<Alg, CP>{});  // marks "done"
+            done[i] = 1;
         }
     }
     // the rest on the device-wide engine, one instance at a time
     ffx_profile acc = ctx->prof;
     for (int32_t i = 0; i < n_inst; ++i) {
-        if (!sched[i].empty()) continue;
-        if (std::find(elig.begin(), elig.end(), i) != elig.end()) ++fallback;
+        if (done[i]) continue;
+        if (ok[i]) ++fallback;
         const ffx_instance& in = inst[i];
         double ex[2] = {0.0, 0.0};
         ctx->prof = ffx_profile{};
