@@ -1230,6 +1230,24 @@ PYBIND11_MODULE(_ffcore, m, py::mod_gil_not_used()) {
             return std::string("MROT(") + (g.r ? "1j " : "") + g.ms.compact() + ", theta=" + std::to_string(g.theta) + ")";
         });
 
+    // ---------------- Jordan-Wigner bridge (src/python_bind.cpp:170-216, Majorana/Pauli part) ----------------
+    m.def("jw", [](const MajoranaString& a) { return jw_poly(mp_string(a)); },
+          "Majorana string to Pauli polynomial (Jordan-Wigner)");
+    m.def("jw", [](const MajoranaPolynomial& p) { return jw_poly(p); },
+          "Majorana polynomial to Pauli polynomial (Jordan-Wigner)");
+    m.def("jw", [](const MRotGate& g) { return jw_rotation(g); },
+          "Majorana rotation to the equal Pauli rotation (Jordan-Wigner)");
+    m.def("topauli", [](const MajoranaString& a) { return jw_poly(mp_string(a)); });
+    m.def("topauli", [](const MajoranaPolynomial& p) { return jw_poly(p); });
+    m.def("tomajorana", [](const PauliString& a) { return majorana_poly(pp_string(a)); },
+          "Pauli string to Majorana polynomial (inverse Jordan-Wigner)");
+    m.def("tomajorana", [](const PauliPolynomial& p) { return majorana_poly(p); },
+          "Pauli polynomial to Majorana polynomial (inverse Jordan-Wigner)");
+    ms.def("topauli", [](const MajoranaString& a) { return jw_poly(mp_string(a)); });
+    mpc.def("topauli", [](const MajoranaPolynomial& p) { return jw_poly(p); });
+    ps.def("tomajorana", [](const PauliString& a) { return majorana_poly(pp_string(a)); });
+    pp.def("tomajorana", [](const PauliPolynomial& p) { return majorana_poly(p); });
+
     // ---------------- propagate (two overloads, Pauli first) ----------------
     m.def("propagate", &propagate_pauli, py::arg("circuit"), py::arg("observable"), py::arg("n_threads") = 1,
           py::arg("maxdegree") = py::none(), py::arg("mincoeff") = py::none(), py::arg("topk") = 0,

@@ -468,4 +468,88 @@ struct MRotGate {
     bool r;
 };
 
+// ---------------------------------------------------------------------------
+// Jordan-Wigner bridge between Majorana and Pauli strings (src/transforms.h:125-160,
+// 254-298), with mode j <-> qubit j and
+//   m_{2j} = Z_0 ... Z_{j-1} X_j,   m_{2j+1} = Z_0 ... Z_{j-1} Y_j.
+// ---------------------------------------------------------------------------
+
+// Image of a normal-ordered Majorana string: the product of the images of its
+// factors in ascending index order; returns the Pauli string, phase in `phase`.
+inline PauliString jw_string(const MajoranaString& a, cplx& phase) {
+    PauliString res;
+    int jpow = 0;
+    for (int i : a.support()) {
+        const int j = i >> 1;
+        if (j >= MAX_QUBITS) fail("Majorana index out of the Pauli range");
+        PauliString f;
+        for (int q = 0; q < j; ++q) f.set(q, 'Z');
+        f.set(j, (i & 1) ? 'Y' : 'X');
+        int jj = 0;
+        res = pauli_mul(res, f, jj);
+        jpow += jj;
+    }
+    phase = ipow(jpow);
+    return res;
+}
+
+// Inverse map.  From the definitions: Z_j = -i m_{2j} m_{2j+1}, so
+// Z_0 ... Z_{j-1} = (-i)^j m_0 m_1 ... m_{2j-1}, X_j = (-i)^j m_0 ... m_{2j-1} m_{2j}
+// and Y_j = (-i)^j m_0 ... m_{2j-1} m_{2j+1}; the images of different qubits
+// commute, so the string is their product in any order.
+inline MajoranaString majorana_of_pauli(const PauliString& p, cplx& phase) {
+    MajoranaString res;
+    int jpow = 0;
+    int sign = 1;
+    for (const auto& [j, op] : p.indices()) {
+        if (2 * j + 1 >= MAX_MAJORANAS) fail("qubit index out of the Majorana range");
+        MajoranaString f;
+        if (op == 'Z') {
+            f.m[(2 * j) >> 6] |= 1ull << ((2 * j) & 63);
+            f.m[(2 * j + 1) >> 6] |= 1ull << ((2 * j + 1) & 63);
+            jpow -= 1;
+        } else {
+            for (int i = 0; i < 2 * j; ++i) f.m[i >> 6] |= 1ull << (i & 63);
+            const int last = op == 'X' ? 2 * j : 2 * j + 1;
+            f.m[last >> 6] |= 1ull << (last & 63);
+            jpow -= j;
+        }
+        int sg = 1;
+        res = majorana_mul(res, f, sg);
+        sign *= sg;
+    }
+    phase = ipow(jpow) * (double)sign;
+    return res;
+}
+
+inline PauliPolynomial jw_poly(const MajoranaPolynomial& a) {
+    PauliPolynomial r;
+    for (size_t i = 0; i < a.t.size(); ++i) {
+        cplx ph;
+        const PauliString ps = jw_string(a.t.keys[i], ph);
+        r.t.add_compress(ps, a.t.vals[i] * ph);
+    }
+    return r;
+}
+
+inline MajoranaPolynomial majorana_poly(const PauliPolynomial& a) {
+    MajoranaPolynomial r;
+    for (size_t i = 0; i < a.t.size(); ++i) {
+        cplx ph;
+        const MajoranaString ms = majorana_of_pauli(a.t.keys[i], ph);
+        r.t.add_compress(ms, a.t.vals[i] * ph);
+    }
+    return r;
+}
+
+// Image of a Majorana rotation exp(-i theta/2 i^r M): i^r M maps to s P with
+// s = +-1 (a Hermitian generator maps to a Hermitian one), i.e. ROT(P, s theta).
+inline RotGate jw_rotation(const MRotGate& g) {
+    cplx ph;
+    const PauliString ps = jw_string(g.ms, ph);
+    const cplx h = (g.r ? cplx(0, 1) : cplx(1, 0)) * ph;
+    if (h.imag() != 0.0 || std::abs(std::abs(h.real()) - 1.0) != 0.0) fail("internal error: non-Hermitian JW image");
+    return RotGate{ps, h.real() > 0 ? g.theta : -g.theta};
+}
+
 }  // namespace ffh

@@ -145,3 +145,55 @@ def batch_kicked_ising(n_inst: int = 512, seed: int = 4242, l_lo: int = 6, l_spa
         x, z, c = single_pauli("Z" * w, sorted(sites), nq)
         out.append((gates, x, z, c, L, depth))
     return out
+
+
+def fermi_hubbard_terms(lx: int, ly: int, t: float = 1.0, u: float = 4.0, pbc: bool = False):
+    """Fermi-Hubbard Hamiltonian -t sum_<ij>,s (a_is^dag a_js + h.c.) + U sum_i n_i,up n_i,dn
+    on an lx x ly lattice as Majorana monomials [(indices, coefficient)], in the
+    Trotter order used by fermi_hubbard (the constant U/4 per site is dropped).
+    Site s = x + lx*y, mode 2s + spin, Majoranas 2*mode and 2*mode + 1 with
+    a_j = (m_2j + i m_2j+1)/2, so n_j = (1 + i m_2j m_2j+1)/2 and
+    a_i^dag a_j + h.c. = (i/2)(m_2i m_2j+1 - m_2i+1 m_2j).  Bonds: horizontal
+    row by row, then vertical; with pbc also the wrap-around bonds."""
+    def site(x, y):
+        return x + lx * y
+
+    bonds = []
+    for y in range(ly):
+        for x in range(lx - 1 + (1 if pbc and lx > 2 else 0)):
+            bonds.append((site(x, y), site((x + 1) % lx, y)))
+    for y in range(ly - 1 + (1 if pbc and ly > 2 else 0)):
+        for x in range(lx):
+            bonds.append((site(x, y), site(x, (y + 1) % ly)))
+    terms = []
+    for (s1, s2) in bonds:
+        a, b = min(s1, s2), max(s1, s2)
+        for spin in (0, 1):
+            i, j = 2 * a + spin, 2 * b + spin
+            terms.append(([2 * i, 2 * j + 1], -0.5j * t))
+            terms.append(([2 * i + 1, 2 * j], 0.5j * t))
+    for s in range(lx * ly):
+        up, dn = 2 * s, 2 * s + 1
+        terms.append(([2 * up, 2 * up + 1], 0.25j * u))
+        terms.append(([2 * dn, 2 * dn + 1], 0.25j * u))
+        terms.append(([2 * up, 2 * up + 1, 2 * dn, 2 * dn + 1], -0.25 * u))
+    return terms
+
+
+def fermi_hubbard(lx: int = 6, ly: int = 6, t: float = 1.0, u: float = 4.0, dt: float = 0.05, steps: int = 20,
+                  pbc: bool = False):
+    """First-order Trotter circuit of fermi_hubbard_terms: per step every
+    monomial c*M in order as MROT(M, c, 2 dt), i.e. theta = Im(c) 2dt for
+    degree 2 (r = 1) and Re(c) 2dt for degree 4 (r = 0).  Returns (packed gate
+    table, Majorana observable n_{0,up} = (1 + i m0 m1)/2 as (m, c))."""
+    step = []
+    for idx, c in fermi_hubbard_terms(lx, ly, t, u, pbc):
+        r = (len(idx) >> 1) & 1
+        step.append((idx, (c.imag if r else c.real) * 2 * dt, r))
+    gates = pack_majorana(step * steps)
+    nmaj = 4 * lx * ly
+    W = words_for_bits(nmaj)
+    m = np.zeros((2, W), np.uint64)
+    m[1, 0] = np.uint64(0b11)
+    c = np.array([0.5, 0.5j])
+    return gates, (m, c)

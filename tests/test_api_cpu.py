@@ -175,3 +175,55 @@ def test_propagate_fails_loudly_without_device():
         ff.propagate([ff.ROT("X", [0], 0.3)], ff.PauliString("Z0"))
     with pytest.raises(RuntimeError, match="no usable HIP device"):
         ff.propagate([ff.MROT(ff.MajoranaString([0, 1]), 0.3)], ff.MajoranaString([0]))
+
+
+def test_jordan_wigner_bridge_vs_dense():
+    """jw / tomajorana (src/transforms.h:125-160, 254-298) against dense
+    Jordan-Wigner matrices on 3 modes, every Majorana string and every Pauli string."""
+    n = 3
+    for r in range(2 * n + 1):
+        for S in itertools.combinations(range(2 * n), r):
+            p = ff.jw(ff.MajoranaString(list(S)))
+            assert np.allclose(pp_dense(p, n), D.majorana_matrix(S, n), atol=1e-14), S
+            back = ff.tomajorana(p)
+            assert back == ff.MajoranaPolynomial(ff.MajoranaString(list(S))), S
+    for s in itertools.product("IXYZ", repeat=n):
+        s = "".join(s)
+        m = ff.PauliString(s).tomajorana()
+        assert np.allclose(mp_dense(m, n), D.pauli_matrix(s), atol=1e-14), s
+    A = ff.poly("0.3 m0 m3 + 1j m1 - 0.5 m0 m1 m2 m5")
+    assert np.allclose(pp_dense(ff.jw(A), n), mp_dense(A, n), atol=1e-14)
+    assert ff.topauli(A) == ff.jw(A) == A.topauli()
+
+
+def test_jordan_wigner_rotation():
+    """The JW image of MROT is the ROT with the same unitary."""
+    n = 3
+    for S, c, t in (([0, 1], 1.0j, 0.3), ([1, 4], -0.5j, 0.7), ([0, 1, 2, 3], -1.0, 0.2), ([2, 5], 2.0j, -0.4),
+                    ([0, 3, 4, 5], 0.25, 1.1)):
+        g = ff.MROT(ff.MajoranaString(S), c, t)
+        r = ff.jw(g)
+        assert np.allclose(pp_dense(r.aspoly(), n), mp_dense(g.aspoly(), n), atol=1e-14), S
+
+
+def test_fermi_hubbard_terms_vs_fermionic_hamiltonian():
+    """The Majorana form of the Hubbard Hamiltonian equals the one built from
+    dense fermionic operators a_j = (m_2j + i m_2j+1)/2 (2x1 lattice, 4 modes),
+    up to the dropped constant U/4 per site."""
+    from fastfermion_amd import workloads as wl
+    lx, ly, t, u = 2, 1, 0.7, 2.3
+    nm = 2 * lx * ly
+    a = [(D.majorana_matrix([2 * j], nm) + 1j * D.majorana_matrix([2 * j + 1], nm)) / 2 for j in range(nm)]
+    H = np.zeros((2 ** nm, 2 ** nm), complex)
+    for s1, s2 in ((0, 1),):
+        for spin in (0, 1):
+            i, j = 2 * s1 + spin, 2 * s2 + spin
+            H += -t * (a[i].conj().T @ a[j] + a[j].conj().T @ a[i])
+    for s in range(lx * ly):
+        nu = a[2 * s].conj().T @ a[2 * s]
+        nd = a[2 * s + 1].conj().T @ a[2 * s + 1]
+        H += u * nu @ nd - u / 4 * np.eye(2 ** nm)
+    M = np.zeros_like(H)
+    for idx, c in wl.fermi_hubbard_terms(lx, ly, t, u):
+        M += c * D.majorana_matrix(idx, nm)
+    assert np.allclose(H, M, atol=1e-13)
