@@ -138,7 +138,7 @@ __device__ __forceinline__ void orbit_gather(const Store<CP, K>& st, const typen
             }
         }
     });
-    uint32_t need_probe = 0;
+    uint32_t need_probe = 0, early = 0;
     sfor<kOrbit>([&](auto TT) {
         constexpr int T = decltype(TT)::value;
         if constexpr (T > 0) {
@@ -147,13 +147,18 @@ __device__ __forceinline__ void orbit_gather(const Store<CP, K>& st, const typen
                 if ((uint32_t)(ent[T] >> 32) == tag[T] && idx < n) {
                     uint64_t kt[NP];
                     orbit_key<T>(k, w, kt);
+                    // the candidate's value travels with its key words (one round less)
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) v[T][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, idx);
                     bool eq = true;
 #pragma unroll
                     for (int q = 0; q < NP; ++q) eq = eq && (st.key[q * cap + idx] == kt[q]);
-                    if (eq)
+                    if (eq) {
                         slot[T] = idx;
-                    else
+                        early |= 1u << T;
+                    } else {
                         need_probe |= 1u << T;
+                    }
                 } else {
                     need_probe |= 1u << T;
                 }
@@ -177,8 +182,11 @@ __device__ __forceinline__ void orbit_gather(const Store<CP, K>& st, const typen
         if constexpr (T > 0) {
             if (((valid >> T) & 1) && slot[T] != kNone) {
                 exist |= 1u << T;
+                if (!((early >> T) & 1)) {
 #pragma unroll
-                for (int kk = 0; kk < K; ++kk) v[T][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, slot[T]);
+                    for (int kk = 0; kk < K; ++kk)
+                        v[T][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, slot[T]);
+                }
                 if (!CP::dead(v[T][0])) alive |= 1u << T;
             }
         }
