@@ -79,6 +79,18 @@ def side_measurements(ctx, gates, W, x, z, c, plain_ms, plain_expect):
     out["pruned"] = {"schedule": "truncation after every rotation (per-gate), FP32 compensated, device-wide engine",
                      "ms": 1e3 * dt, "abs_err_expectation": abs(e - plain_expect),
                      "speedup_vs_plain_fp64": plain_ms / (1e3 * dt)}
+    # Majorana path at the Fermi-Hubbard size: 6x6 lattice, 72 modes (144 Majoranas, 3 words),
+    # t=1, U=4, dt=0.05, 20 Trotter steps (6960 rotations), observable n_{0,up}, read out on
+    # the Neel-like Fock state with the up spins of the even sites occupied
+    fh, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20)
+    occ = sum(1 << (2 * s) for s in range(36) if s % 2 == 0)
+    dt, (_, efh) = _time(lambda: ctx.majorana_propagate(fh, 3, fm, fc, maxdegree=8, mincoeff=1e-4, download=False,
+                                                        occ=[occ & ((1 << 64) - 1), occ >> 64]))
+    p = ctx.profile()
+    out["majorana"] = {"workload": "Fermi-Hubbard 6x6 (72 modes), t=1, U=4, dt=0.05, 20 Trotter steps, 6960 "
+                       "rotations, observable n_{0,up}, maxdegree 8, mincoeff 1e-4, FP64", "ms": 1e3 * dt,
+                       "updates": int(p["updates"]), "updates_per_s": p["updates"] / dt,
+                       "peak_live_terms": int(p["peak_terms"]), "expectation_neel": efh.real}
     inst = wl.batch_kicked_ising(512)
     Wb = max(_abi.words_for_bits(L * L) for (_, _, _, _, L, _) in inst)
 
