@@ -118,7 +118,7 @@ def main():
     a = parse()
     if a.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for --gpus {a.gpus}); "
-              "the multi-GPU sharding is described in DESIGN.md and not implemented", file=sys.stderr)
+              "the multi-GPU sharding by key hash is a design only and is not implemented", file=sys.stderr)
         sys.exit(2)
     import numpy as np
 

@@ -1288,6 +1288,16 @@ device (HIP device index), precision ('fp64', 'fp32', 'fp32c').)DOC");
         d["propagate"] = p.seconds_propagate;
         d["download"] = p.seconds_download;
         d["total"] = p.seconds_total;
+        // the reference's keys (src/python_bind.cpp:433-446): its CPU phases map to
+        // the device ones; emission and merging are one kernel here, and a
+        // single device routes every partner locally
+        d["snapshot"] = p.seconds_upload;
+        d["emit"] = p.seconds_propagate;
+        d["merge"] = 0.0;
+        d["emit_per_thread"] = py::list();
+        d["merge_per_thread"] = py::list();
+        d["n_local"] = 0;
+        d["n_remote"] = 0;
         return d;
     });
     m.def("device_count", []() { return ffx_device_count(); });

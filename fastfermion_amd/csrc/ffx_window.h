@@ -333,23 +333,55 @@ __device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP
         if (threadIdx.x == 0) *sbase = atomicAdd(sink.ctr, tot);
         __syncthreads();
         if (napp) {
-            uint64_t at = (uint64_t)n + *sbase + rank;
+            // store every fresh member, then claim all their home index entries at
+            // once (independent CAS round trips in flight together); only the
+            // members whose home entry was taken walk the probe sequence
+            const uint64_t at0 = (uint64_t)n + *sbase + rank;
+            const size_t cap = st.cap;
+            uint64_t prev[kOrbit];
             sfor<kOrbit>([&](auto TT) {
                 constexpr int T = decltype(TT)::value;
                 if constexpr (T > 0) {
                     if ((fresh >> T) & 1) {
-                        if (at < st.cap) {
+                        const uint64_t at = at0 + __popc(fresh & ((1u << T) - 1u));
+                        if (at < cap) {
                             uint64_t kt[NP];
                             orbit_key<T>(k, w, kt);
-                            put_slot_h<NP, CP, K>(st, st.key, cout, at, kt, v[T], hmix(lk ^ lp_of<T>(w)));
+#pragma unroll
+                            for (int q = 0; q < NP; ++q) st.key[q * cap + at] = kt[q];
+#pragma unroll
+                            for (int kk = 0; kk < K; ++kk)
+                                CP::store(cout + (size_t)kk * CP::NS * cap, cap, at, v[T][kk]);
+                            const uint64_t h = hmix(lk ^ lp_of<T>(w));
+                            const uint64_t e = ((h >> 32) << 32) | at;
+                            prev[T] = (uint64_t)atomicCAS((unsigned long long*)&st.index[h & st.hmask],
+                                                          (unsigned long long)kEmpty, (unsigned long long)e);
                         } else {
+                            prev[T] = kEmpty;
                             atomicOr(sink.flag, 1u);
                             if (sink.flag_epoch) *sink.flag_epoch = sink.e;
                         }
-                        ++at;
                     }
                 }
             });
+            uint32_t retry = 0;
+            sfor<kOrbit>([&](auto TT) {
+                constexpr int T = decltype(TT)::value;
+                if constexpr (T > 0)
+                    if (((fresh >> T) & 1) && prev[T] != kEmpty) retry |= 1u << T;
+            });
+            if (retry) {
+                sfor<kOrbit>([&](auto TT) {
+                    constexpr int T = decltype(TT)::value;
+                    if constexpr (T > 0) {
+                        if ((retry >> T) & 1) {
+                            const uint64_t at = at0 + __popc(fresh & ((1u << T) - 1u));
+                            index_insert(st.index, st.hmask, hmix(lk ^ lp_of<T>(w)), (uint32_t)at,
+                                         &st.ctrl->index_full);
+                        }
+                    }
+                });
+            }
         }
     }
 }

@@ -204,3 +204,18 @@ def test_majorana_truncation_through_api():
     for occ in ([], [0], [1, 3]):
         want = pyffo.majorana_expect(ref, [sum(1 << q for q in occ)])
         assert abs(ff.FockState(occ)(res) - want) < 1e-12
+
+
+def test_profile_dictionary_has_reference_keys():
+    """profile_reset / profile_get (src/python_bind.cpp:427-446): the reference's
+    keys are present, plus the device counters."""
+    ff.profile_reset()
+    circ = tfim(6, 0.1) * 3
+    propagate(circ, ff.PauliString("Z" + "I" * 5))
+    d = ff.profile_get()
+    for k in ("snapshot", "emit", "merge", "n_gates", "emit_per_thread", "merge_per_thread", "n_local", "n_remote",
+              "updates", "peak_terms"):
+        assert k in d, k
+    assert d["n_gates"] == len(circ)
+    assert d["updates"] > 0 and d["emit"] >= 0.0
+    ff.profile_disable()
