@@ -23,7 +23,13 @@ STATUS = {0: "ok", -1: "invalid argument", -2: "out of memory", -3: "HIP error",
 
 EXPORTED = ("ffx_ctx_create", "ffx_ctx_destroy", "ffx_last_error", "ffx_pauli_propagate",
             "ffx_majorana_propagate", "ffx_expect_last", "ffx_terms_free", "ffx_profile_get",
-            "ffx_device_count", "ffx_set_kernel_timing", "ffx_pauli_propagate_batch")
+            "ffx_device_count", "ffx_set_kernel_timing", "ffx_pauli_propagate_batch",
+            "ffx_pauli_propagate_pruned", "ffx_majorana_propagate_pruned", "ffx_pauli_contrib_log",
+            "ffx_majorana_contrib_log", "ffx_pruner_init", "ffx_pruner_set_params", "ffx_pruner_get_params",
+            "ffx_pruner_forward", "ffx_pruner_loss_grad", "ffx_pruner_train")
+
+PRUNER_FEATURES = 64
+PRUNER_NPARAMS = 24961
 
 
 class Options(ctypes.Structure):
@@ -31,6 +37,27 @@ class Options(ctypes.Structure):
                 ("max_xweight", ctypes.c_int32), ("xtrunc_period", ctypes.c_int32),
                 ("batched", ctypes.c_int32), ("precision", ctypes.c_int32),
                 ("mincoeff", ctypes.c_double)]
+
+
+class PruneOpts(ctypes.Structure):
+    """ffx_prune_opts: pruned-mode switches and the site geometry of the term features."""
+    _fields_ = [("clifford_tol", ctypes.c_double), ("log10_tau", ctypes.c_float),
+                ("prune_period", ctypes.c_int32), ("rot_per_layer", ctypes.c_int32),
+                ("obs_site", ctypes.c_int32), ("row", ctypes.c_uint8 * 256), ("col", ctypes.c_uint8 * 256)]
+
+
+def prune_opts(row, col, obs_site=0, clifford_tol=0.0, prune_period=0, log10_tau=-30.0, rot_per_layer=1):
+    """PruneOpts from per-index lattice rows/columns (qubits, or Majorana indices)."""
+    po = PruneOpts()
+    po.clifford_tol = float(clifford_tol)
+    po.log10_tau = float(log10_tau)
+    po.prune_period = int(prune_period)
+    po.rot_per_layer = int(rot_per_layer)
+    po.obs_site = int(obs_site)
+    for i, (r, c) in enumerate(zip(row, col)):
+        po.row[i] = int(r)
+        po.col[i] = int(c)
+    return po
 
 
 class Terms(ctypes.Structure):
@@ -83,6 +110,22 @@ def load():
             L.ffx_set_kernel_timing.argtypes = [vp, ctypes.c_int]
             L.ffx_pauli_propagate_batch.argtypes = [vp, ctypes.POINTER(Instance), i32, i32, ctypes.POINTER(Options),
                                                     i64, vp, vp, vp]
+            PO = ctypes.POINTER(PruneOpts)
+            L.ffx_pauli_propagate_pruned.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i64, ctypes.POINTER(Options),
+                                                     PO, ctypes.POINTER(Terms), vp, vp]
+            L.ffx_majorana_propagate_pruned.argtypes = [vp, vp, i64, i32, vp, vp, vp, i64, i32, dbl, i32, PO,
+                                                        ctypes.POINTER(Terms), vp, vp]
+            L.ffx_pauli_contrib_log.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i64, ctypes.POINTER(Options), PO, vp,
+                                                i32, i64, vp, vp, vp, vp, i32, vp]
+            L.ffx_majorana_contrib_log.argtypes = [vp, vp, i64, i32, vp, vp, vp, i64, i32, dbl, PO, vp, i32, i64, vp,
+                                                   vp, vp, vp, i32, vp]
+            L.ffx_pruner_init.argtypes = [vp, ctypes.c_uint64]
+            L.ffx_pruner_set_params.argtypes = [vp, vp, i64]
+            L.ffx_pruner_get_params.argtypes = [vp, vp, i64]
+            L.ffx_pruner_forward.argtypes = [vp, vp, i64, vp]
+            L.ffx_pruner_loss_grad.argtypes = [vp, vp, vp, i64, ctypes.c_float, vp, vp]
+            L.ffx_pruner_train.argtypes = [vp, vp, vp, i64, i32, i32, ctypes.c_float, ctypes.c_float, ctypes.c_uint64,
+                                           vp]
             _lib = L
     return _lib
 
@@ -234,6 +277,160 @@ class Context:
         if want_expect or occ is not None:
             return res, complex(ex[0], ex[1])
         return res
+
+    # ---- pruned mode, contribution logs, learned pruner ----
+    @staticmethod
+    def _opts(maxdegree, topk, max_xweight, xtrunc_period, batched, precision, mincoeff):
+        return Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+
+    def pauli_propagate_pruned(self, gates, n_words, x, z, c, po, maxdegree=-1, mincoeff=0.0, topk=0, max_xweight=-1,
+                               xtrunc_period=1, batched=True, precision=FP64, download=True, state=None,
+                               want_expect=False):
+        """pauli_propagate with the pruned mode of `po` (a PruneOpts)."""
+        gates = np.ascontiguousarray(gates, dtype=PAULI_GATE_DTYPE)
+        x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, n_words)
+        z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, n_words)
+        c = np.asarray(c, dtype=np.complex128).reshape(-1)
+        re = np.ascontiguousarray(c.real)
+        im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+        opts = self._opts(maxdegree, topk, max_xweight, xtrunc_period, batched, precision, mincoeff)
+        t = Terms()
+        ex = np.zeros(2)
+        st = None
+        if state is not None:
+            st = np.zeros(4, np.uint64)
+            st[: len(state)] = state
+        rc = self._L.ffx_pauli_propagate_pruned(self._h, _p(gates), len(gates), n_words, _p(x), _p(z), _p(re),
+                                                _p(im), len(c), ctypes.byref(opts),
+                                                ctypes.byref(po) if po is not None else None,
+                                                ctypes.byref(t) if download else None, _p(st),
+                                                _p(ex) if (want_expect or state is not None) else None)
+        self._check(rc)
+        res = None
+        if download:
+            try:
+                res = self._collect(t, True)
+            finally:
+                self._L.ffx_terms_free(ctypes.byref(t))
+        if want_expect or state is not None:
+            return res, complex(ex[0], ex[1])
+        return res
+
+    def majorana_propagate_pruned(self, gates, n_words, m, c, po, maxdegree=-1, mincoeff=0.0, precision=FP64,
+                                  download=True, occ=None, want_expect=False):
+        gates = np.ascontiguousarray(gates, dtype=MAJORANA_GATE_DTYPE)
+        m = np.ascontiguousarray(m, dtype=np.uint64).reshape(-1, n_words)
+        c = np.asarray(c, dtype=np.complex128).reshape(-1)
+        re = np.ascontiguousarray(c.real)
+        im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+        t = Terms()
+        ex = np.zeros(2)
+        oc = None
+        if occ is not None:
+            oc = np.zeros(4, np.uint64)
+            oc[: len(occ)] = occ
+        rc = self._L.ffx_majorana_propagate_pruned(self._h, _p(gates), len(gates), n_words, _p(m), _p(re), _p(im),
+                                                   len(c), maxdegree, mincoeff, precision,
+                                                   ctypes.byref(po) if po is not None else None,
+                                                   ctypes.byref(t) if download else None, _p(oc),
+                                                   _p(ex) if (want_expect or occ is not None) else None)
+        self._check(rc)
+        res = None
+        if download:
+            try:
+                res = self._collect(t, False)
+            finally:
+                self._L.ffx_terms_free(ctypes.byref(t))
+        if want_expect or occ is not None:
+            return res, complex(ex[0], ex[1])
+        return res
+
+    def _log_out(self, max_rows, n_points):
+        feats = np.zeros((max(max_rows, 1), PRUNER_FEATURES), np.float32)
+        targ = np.zeros(max(max_rows, 1), np.float32)
+        sums = np.zeros(2 * n_points)
+        return feats, targ, sums
+
+    def pauli_contrib_log(self, gates, n_words, x, z, c, po, log_every=1, max_rows=1 << 20, maxdegree=-1,
+                          mincoeff=0.0, max_xweight=-1, xtrunc_period=1, batched=True, state=None):
+        """Term-contribution log of one plain FP64 propagation: (features float32[n, 64],
+        targets float32[n] = clamp(log10(|v| + 1e-12), -12, 0)/4, sums complex[points]
+        = sum_q c_q v_q at every logged step)."""
+        gates = np.ascontiguousarray(gates, dtype=PAULI_GATE_DTYPE)
+        x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, n_words)
+        z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, n_words)
+        c = np.asarray(c, dtype=np.complex128).reshape(-1)
+        re = np.ascontiguousarray(c.real)
+        im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+        opts = self._opts(maxdegree, 0, max_xweight, xtrunc_period, batched, FP64, mincoeff)
+        npts = len(gates) // max(log_every, 1) + 2
+        feats, targ, sums = self._log_out(max_rows, npts)
+        st = np.zeros(4, np.uint64)
+        if state is not None:
+            st[: len(state)] = state
+        nr, npo = ctypes.c_int64(0), ctypes.c_int32(0)
+        self._check(self._L.ffx_pauli_contrib_log(self._h, _p(gates), len(gates), n_words, _p(x), _p(z), _p(re),
+                                                  _p(im), len(c), ctypes.byref(opts), ctypes.byref(po), _p(st),
+                                                  log_every, max_rows, _p(feats), _p(targ), ctypes.byref(nr),
+                                                  _p(sums), npts, ctypes.byref(npo)))
+        n, k = nr.value, npo.value
+        return feats[:n], targ[:n], sums[0:2 * k:2] + 1j * sums[1:2 * k:2]
+
+    def majorana_contrib_log(self, gates, n_words, m, c, po, log_every=1, max_rows=1 << 20, maxdegree=-1,
+                             mincoeff=0.0, occ=None):
+        gates = np.ascontiguousarray(gates, dtype=MAJORANA_GATE_DTYPE)
+        m = np.ascontiguousarray(m, dtype=np.uint64).reshape(-1, n_words)
+        c = np.asarray(c, dtype=np.complex128).reshape(-1)
+        re = np.ascontiguousarray(c.real)
+        im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+        npts = len(gates) // max(log_every, 1) + 2
+        feats, targ, sums = self._log_out(max_rows, npts)
+        oc = np.zeros(4, np.uint64)
+        if occ is not None:
+            oc[: len(occ)] = occ
+        nr, npo = ctypes.c_int64(0), ctypes.c_int32(0)
+        self._check(self._L.ffx_majorana_contrib_log(self._h, _p(gates), len(gates), n_words, _p(m), _p(re), _p(im),
+                                                     len(c), maxdegree, mincoeff, ctypes.byref(po), _p(oc), log_every,
+                                                     max_rows, _p(feats), _p(targ), ctypes.byref(nr), _p(sums), npts,
+                                                     ctypes.byref(npo)))
+        n, k = nr.value, npo.value
+        return feats[:n], targ[:n], sums[0:2 * k:2] + 1j * sums[1:2 * k:2]
+
+    def pruner_init(self, seed: int = 1) -> None:
+        self._check(self._L.ffx_pruner_init(self._h, ctypes.c_uint64(seed)))
+
+    def pruner_set_params(self, params) -> None:
+        p = np.ascontiguousarray(params, dtype=np.float32).reshape(-1)
+        self._check(self._L.ffx_pruner_set_params(self._h, _p(p), len(p)))
+
+    def pruner_get_params(self):
+        p = np.zeros(PRUNER_NPARAMS, np.float32)
+        self._check(self._L.ffx_pruner_get_params(self._h, _p(p), len(p)))
+        return p
+
+    def pruner_forward(self, feats):
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, PRUNER_FEATURES)
+        out = np.zeros(len(f), np.float32)
+        self._check(self._L.ffx_pruner_forward(self._h, _p(f), len(f), _p(out)))
+        return out
+
+    def pruner_loss_grad(self, feats, targets, under_weight=1.0):
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, PRUNER_FEATURES)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        g = np.zeros(PRUNER_NPARAMS, np.float32)
+        loss = ctypes.c_double(0.0)
+        self._check(self._L.ffx_pruner_loss_grad(self._h, _p(f), _p(t), len(f), ctypes.c_float(under_weight),
+                                                 ctypes.byref(loss), _p(g)))
+        return loss.value, g
+
+    def pruner_train(self, feats, targets, epochs=4, batch=4096, lr=1e-3, seed=7, under_weight=1.0):
+        """Adam on the device; returns the loss of every epoch."""
+        f = np.ascontiguousarray(feats, dtype=np.float32).reshape(-1, PRUNER_FEATURES)
+        t = np.ascontiguousarray(targets, dtype=np.float32).reshape(-1)
+        hist = np.zeros(max(epochs, 1))
+        self._check(self._L.ffx_pruner_train(self._h, _p(f), _p(t), len(f), epochs, batch, ctypes.c_float(lr),
+                                             ctypes.c_float(under_weight), ctypes.c_uint64(seed), _p(hist)))
+        return hist[:epochs]
 
     def expect_last(self, state=None) -> complex:
         st = np.zeros(4, np.uint64)

@@ -26,8 +26,34 @@
 
 #include "../../include/fastfermion_amd.h"
 #include "ffx_batch.h"
+#include "ffx_prune.h"
 
 using namespace ffx;
+
+// Device state of a context's pruner model (allocated on first use).
+struct MlpState {
+    float* p = nullptr;    // fp32 master parameters (FFX_PRUNER_NPARAMS)
+    float* g = nullptr;    // gradient
+    float* m1 = nullptr;   // Adam moments
+    float* m2 = nullptr;
+    uint16_t* w1t = nullptr;  // bf16 W1^T [128][64]
+    uint16_t* w2t = nullptr;  // bf16 W2^T [128][128]
+    uint16_t* w2 = nullptr;   // bf16 W2 [128][128]
+    float* w3r = nullptr;     // w3 rounded to bf16
+    double* loss = nullptr;   // accumulated squared error
+    int64_t step = 0;         // Adam steps taken
+    bool ready = false;       // parameters set
+    // training data and minibatch buffers
+    uint16_t* X = nullptr;
+    float* T = nullptr;
+    size_t data_cap = 0;
+    int32_t* idx = nullptr;
+    size_t idx_cap = 0;
+    uint16_t *H1 = nullptr, *DZ1 = nullptr, *DZ2 = nullptr;
+    float* Y = nullptr;
+    size_t mb_cap = 0;
+    MlpDev dev() const { return MlpDev{w1t, w2t, w2, w3r, p + kOffB1, p + kOffB2, p + kOffB3}; }
+};
 
 struct ffx_ctx {
     int device = 0;
@@ -69,7 +95,13 @@ struct ffx_ctx {
     int last_W = 0, last_K = 0, last_prec = 0;
     ffx_profile prof{};
     int64_t launches = 0;
+    MlpState* mlp = nullptr;  // pruner model (ffx_pruner_*)
+    void* crot = nullptr;     // Clifford-rotation records of the pruned mode
+    size_t crot_bytes = 0;
 };
+
+// Releases the pruner model of a context (defined with the pruner kernels).
+void ffx_mlp_free(ffx_ctx* ctx);
 
 namespace ffx_host {
 
@@ -537,7 +569,7 @@ struct Engine {
 
     // ---- asynchronous engine: one launch per step, no host sync between steps ----
     struct AStep {
-        int kind = 0;  // 0 rotation, 1 Clifford block
+        int kind = 0;  // 0 rotation, 1 Clifford block, 2 run of Clifford rotations (pruned mode)
         typename Alg::Gate g{};
         double cs = 1.0, sn = 0.0;
         int trunc_thresh = 0, trunc_xw = 0;
@@ -545,6 +577,9 @@ struct Engine {
         int topk_after = 0;     // top-k truncation after this step
         int topk_xw = 0;        // ... followed by the X/Y-weight cap
         int cb = 0, cn = 0;     // Clifford gates [cb, cb+cn) of the block list
+        int rb = 0, rn = 0;     // kind 2: Clifford rotations [rb, rb+rn) of the record list
+        int prune_after = 0;    // learned pruning after this step
+        FeatCtx fctx{};         // feature context of that pruning
     };
     uint32_t* ep_buf = nullptr;  // owned by ctx (ctx->epochs)
     int64_t st_updates = 0, st_outs = 0, st_peak = 0, st_rot = 0, st_compact = 0;
@@ -614,7 +649,9 @@ struct Engine {
         size_t k = 1;
         while ((int)k < max_window && t + k < steps.size()) {
             const AStep& nx = steps[t + k];
-            if (steps[t + k - 1].compact_after || steps[t + k - 1].topk_after || nx.kind != 0) break;
+            if (steps[t + k - 1].compact_after || steps[t + k - 1].topk_after || steps[t + k - 1].prune_after ||
+                nx.kind != 0)
+                break;
             bool ok = true;
             for (size_t j = 0; j < k && ok; ++j) ok = axes_commute(steps[t + j].g, nx.g);
             if (!ok || !add(nx.g.p)) break;
@@ -656,8 +693,26 @@ struct Engine {
     }
 
     int run_async(const std::vector<AStep>& steps, const std::vector<CliffGate>& cliff, double mincoeff, int maxdeg,
-                  int max_xw, uint32_t topk = 0) {
+                  int max_xw, uint32_t topk = 0, const std::vector<CRot<Alg, CP>>* crots = nullptr,
+                  const PruneGeo* geo = nullptr, float log10_tau = 0.f) {
         if (steps.empty()) return FFX_OK;
+        if (crots && !crots->empty()) {
+            const size_t bytes = crots->size() * sizeof(CRot<Alg, CP>);
+            if (bytes > ctx->crot_bytes) {
+                dev_free(ctx->crot);
+                ctx->crot_bytes = 0;
+                CHK(dev_alloc(ctx, &ctx->crot, bytes));
+                ctx->crot_bytes = bytes;
+            }
+            HIPCHK(hipMemcpyAsync(ctx->crot, crots->data(), bytes, hipMemcpyHostToDevice, ctx->stream));
+        }
+        bool pruning = false;
+        for (const AStep& a : steps) pruning = pruning || a.prune_after;
+        if (pruning) {
+            if (K != 1) return set_err(ctx, FFX_EINVAL, "the learned pruner needs real coefficients");
+            if (!ctx->mlp || !ctx->mlp->ready || !geo)
+                return set_err(ctx, FFX_EINVAL, "the learned pruner has no model (ffx_pruner_init / set_params)");
+        }
         if (!cliff.empty()) {
             if (cliff.size() > ctx->cliff_cap) {
                 dev_free(ctx->cliff);
@@ -685,7 +740,8 @@ struct Engine {
             // epochs needed from step s on
             size_t ne = 1;
             for (size_t t = s; t < steps.size(); ++t)
-                ne += 1 + (steps[t].compact_after ? 1 : 0) + (steps[t].topk_after ? 1 : 0);
+                ne += 1 + (steps[t].compact_after ? 1 : 0) + (steps[t].topk_after ? 1 : 0) +
+                      (steps[t].prune_after ? 1 : 0);
             CHK(ensure_epochs(ne));
             CHK(ensure_counts(ctx->cap / kTile + 2));
             const Epochs ep = epochs();
@@ -698,7 +754,7 @@ struct Engine {
             HIPCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
             HIPCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
             std::vector<int64_t> step_of(ne, -1);
-            std::vector<int> is_rot(ne, 0);
+            std::vector<int> is_rot(ne, 0), crot_len(ne, 0);
             std::vector<std::pair<void*, void*>> planes_at(steps.size());
             const unsigned g_cap = grid_for(ctx->cap, kTile, 4096);
             const unsigned g_idx = grid_for(std::max(ctx->cap, ctx->hslots), kBlock, 2048);
@@ -750,6 +806,16 @@ struct Engine {
                     step_of[e] = (int64_t)t;
                     is_rot[e] = 1;
                     ++e;
+                } else if (a.kind == 2) {
+                    hipLaunchKernelGGL((k_crot_async<Alg, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream, store(),
+                                       (const CRot<Alg, CP>*)ctx->crot + a.rb, a.rn, mincoeff, maxdeg, max_xw, ep, e);
+                    CHK(launch_check());
+                    hipLaunchKernelGGL(k_index_build_async<NP>, dim3(g_idx), dim3(kBlock), 0, ctx->stream, ctx->key,
+                                       ctx->cap, ctx->index, ctx->hslots - 1, ep, e, e, 0, &ctx->ctrl->index_full);
+                    CHK(launch_check());
+                    step_of[e] = (int64_t)t;
+                    crot_len[e] = a.rn;
+                    ++e;
                 } else {
                     if constexpr (Alg::kPauli) {
                         hipLaunchKernelGGL((k_cliff_async<Alg::W, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream,
@@ -762,6 +828,17 @@ struct Engine {
                     }
                     step_of[e] = (int64_t)t;
                     ++e;
+                }
+                if (steps[t + kw - 1].prune_after) {
+                    if constexpr (K == 1) {
+                        const unsigned g_pr = (unsigned)std::max(1, 2 * ctx->n_cu);
+                        hipLaunchKernelGGL((k_prune_async<Alg, CP>), dim3(g_pr), dim3(64 * kPruneWaves), 0,
+                                           ctx->stream, store(), ep, e, ctx->mlp->dev(), *geo,
+                                           steps[t + kw - 1].fctx, log10_tau);
+                        CHK(launch_check());
+                        step_of[e] = (int64_t)t;
+                        ++e;
+                    }
                 }
                 if (steps[t + kw - 1].topk_after) {
                     const uint32_t ep_prev = e - 1;
@@ -824,6 +901,19 @@ struct Engine {
             }
             const uint32_t e_end = hf[0] ? hf[1] : e;
             for (uint32_t q = 1; q < e_end; ++q) {
+                if (crot_len[q]) {
+                    // a run of Clifford rotations: the kernel sums, over the terms live at its
+                    // start, the rotations each one survives; a term erased inside the run also
+                    // entered the rotation that erased it
+                    uint64_t surv = 0;
+                    std::memcpy(&surv, &hsd[(size_t)kWin * q], sizeof(surv));
+                    const int64_t lout = (int64_t)(hn[q] + ha[q]) - (int64_t)((int32_t)hd[q] + hdd[q]);
+                    st_rot += crot_len[q];
+                    st_updates += (int64_t)surv + hdd[q];
+                    st_outs += (int64_t)surv;
+                    st_peak = std::max<int64_t>(st_peak, lout);
+                    continue;
+                }
                 if (!is_rot[q]) {
                     if (hn[q] != hn[q - 1] + ha[q - 1]) st_compact++;
                     continue;
@@ -942,6 +1032,107 @@ typename MajoranaAlg<W>::Gate majorana_gate_dev(const ffx_majorana_gate& g) {
     return d;
 }
 
+// Site geometry of the term features from the pruned-mode options.
+inline PruneGeo prune_geo(const ffx_prune_opts& po) {
+    PruneGeo g{};
+    std::memcpy(g.row, po.row, sizeof(g.row));
+    std::memcpy(g.col, po.col, sizeof(g.col));
+    const int s = (po.obs_site >= 0 && po.obs_site < 256) ? po.obs_site : 0;
+    g.obs_row = po.row[s];
+    g.obs_col = po.col[s];
+    return g;
+}
+
+// Feature context after `done` of `nrot` rotations: remaining layers, mean |theta|
+// of the remaining branching rotations (|cos theta| > 1e-6), fraction done.
+struct FeatPlan {
+    std::vector<double> suf_sum;
+    std::vector<int> suf_cnt;
+    size_t nrot = 0;
+    int rot_per_layer = 1;
+    template <class AStep>
+    FeatPlan(const std::vector<AStep>& sd, int rpl) : rot_per_layer(rpl > 0 ? rpl : 1) {
+        for (const AStep& a : sd) nrot += a.kind == 0;
+        suf_sum.assign(nrot + 1, 0.0);
+        suf_cnt.assign(nrot + 1, 0);
+        size_t r = nrot;
+        for (size_t t = sd.size(); t-- > 0;) {
+            if (sd[t].kind != 0) continue;
+            --r;
+            const bool br = std::fabs(sd[t].cs) > 1e-6;
+            suf_sum[r] = suf_sum[r + 1] + (br ? std::fabs(std::atan2(sd[t].sn, sd[t].cs)) : 0.0);
+            suf_cnt[r] = suf_cnt[r + 1] + (br ? 1 : 0);
+        }
+    }
+    FeatCtx at(size_t done) const {
+        FeatCtx f{};
+        f.rem_layers = (float)(nrot - done) / (float)rot_per_layer;
+        f.mean_theta = suf_cnt[done] ? (float)(suf_sum[done] / suf_cnt[done]) : 0.f;
+        f.frac_done = nrot ? (float)done / (float)nrot : 0.f;
+        return f;
+    }
+};
+
+// Pruned-mode plan over a list of device steps: rotations with |cos theta| <=
+// clifford_tol become records of runs of Clifford rotations (consecutive ones
+// share one kind-2 step); every prune_period-th rotation is followed by a
+// learned-pruning epoch.
+template <class Alg, class CP, int K>
+void prune_plan(std::vector<typename Engine<Alg, CP, K>::AStep>& sd, std::vector<CRot<Alg, CP>>& crots,
+                const ffx_prune_opts& po) {
+    using AStep = typename Engine<Alg, CP, K>::AStep;
+    const FeatPlan fp(sd, po.rot_per_layer);
+    std::vector<AStep> out;
+    out.reserve(sd.size());
+    size_t done = 0, compacts = 0;
+    size_t thin = 4;  // keep every thin-th compaction point
+    if (const char* ev = getenv("FFX_PRUNE_COMPACT"))
+        if (atoi(ev) > 0) thin = (size_t)atoi(ev);
+    for (AStep a : sd) {
+        if (a.kind != 0) {
+            out.push_back(a);
+            continue;
+        }
+        ++done;
+        // pruned runs keep few terms: a quarter of the compaction points is enough
+        if (a.compact_after) a.compact_after = (compacts++ % thin) == thin - 1 ? 1 : 0;
+        const bool cliff = po.clifford_tol > 0 && std::fabs(a.cs) <= po.clifford_tol;
+        if (cliff) {
+            CRot<Alg, CP> cr{};
+            cr.g = a.g;
+            cr.sn = make_fac<CP>(a.sn);
+            cr.thresh = a.trunc_thresh;
+            cr.xw = a.trunc_xw;
+            AStep* open = (!out.empty() && out.back().kind == 2 && !out.back().topk_after && !out.back().prune_after)
+                              ? &out.back()
+                              : nullptr;
+            if (open) {
+                open->rn++;
+                open->compact_after |= a.compact_after;
+                open->topk_after = a.topk_after;
+                open->topk_xw = a.topk_xw;
+            } else {
+                AStep g2{};
+                g2.kind = 2;
+                g2.rb = (int)crots.size();
+                g2.rn = 1;
+                g2.compact_after = a.compact_after;
+                g2.topk_after = a.topk_after;
+                g2.topk_xw = a.topk_xw;
+                out.push_back(g2);
+            }
+            crots.push_back(cr);
+        } else {
+            out.push_back(a);
+        }
+        if (po.prune_period > 0 && done % (size_t)po.prune_period == 0) {
+            out.back().prune_after = 1;
+            out.back().fctx = fp.at(done);
+        }
+    }
+    sd.swap(out);
+}
+
 // Device steps of the asynchronous engine for a Pauli schedule: rotation
 // parameters, truncation points, top-k points, compaction points, Clifford blocks.
 template <int W, class CP, int K>
@@ -983,7 +1174,7 @@ void pauli_async_steps(const ffx_pauli_gate* gates, const std::vector<Step>& ste
 template <int W, class CP, int K>
 int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const uint64_t* x, const uint64_t* z,
               const double* re, const double* im, int64_t n_terms, const ffx_options& o, ffx_terms* out,
-              const uint64_t* state, double* out_expect) {
+              const uint64_t* state, double* out_expect, const ffx_prune_opts* po = nullptr) {
     using Alg = PauliAlg<W>;
     Engine<Alg, CP, K> eng(ctx);
     const double t0 = now_s();
@@ -996,7 +1187,14 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
         std::vector<AStep> sd;
         std::vector<CliffGate> cl;
         pauli_async_steps<W, CP, K>(gates, steps, o, sd, cl);
-        CHK(eng.run_async(sd, cl, o.mincoeff, maxdeg, o.max_xweight, o.topk > 0 ? (uint32_t)o.topk : 0u));
+        std::vector<CRot<Alg, CP>> crots;
+        PruneGeo geo{};
+        if (po) {
+            prune_plan<Alg, CP, K>(sd, crots, *po);
+            geo = prune_geo(*po);
+        }
+        CHK(eng.run_async(sd, cl, o.mincoeff, maxdeg, o.max_xweight, o.topk > 0 ? (uint32_t)o.topk : 0u, &crots,
+                          &geo, po ? po->log10_tau : 0.f));
     } else
     for (const Step& s : steps) {
         if (s.kind == 0) {
@@ -1042,7 +1240,7 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
 template <int W, class CP, int K>
 int run_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, const uint64_t* m, const double* re,
                  const double* im, int64_t n_terms, int maxdegree, double mincoeff, ffx_terms* out,
-                 const uint64_t* occ, double* out_expect) {
+                 const uint64_t* occ, double* out_expect, const ffx_prune_opts* po = nullptr) {
     using Alg = MajoranaAlg<W>;
     Engine<Alg, CP, K> eng(ctx);
     const double t0 = now_s();
@@ -1063,7 +1261,13 @@ int run_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, 
             d.compact_after = (mincoeff > 0 && (++cnt % 16) == 0) ? 1 : 0;
             sd.push_back(d);
         }
-        CHK(eng.run_async(sd, {}, mincoeff, maxdeg, -1));
+        std::vector<CRot<Alg, CP>> crots;
+        PruneGeo geo{};
+        if (po) {
+            prune_plan<Alg, CP, K>(sd, crots, *po);
+            geo = prune_geo(*po);
+        }
+        CHK(eng.run_async(sd, {}, mincoeff, maxdeg, -1, 0u, &crots, &geo, po ? po->log10_tau : 0.f));
     } else
     for (int64_t i = n_gates - 1; i >= 0; --i) {
         const ffx_majorana_gate& g = gates[i];
@@ -1311,35 +1515,260 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
 template <int W>
 int dispatch_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const uint64_t* x, const uint64_t* z,
                    const double* re, const double* im, int64_t n_terms, const ffx_options& o, ffx_terms* out,
-                   const uint64_t* state, double* out_expect, int prec, int K) {
-    if (prec == FFX_FP64)
-        return K == 1 ? run_pauli<W, F64, 1>(ctx, gates, n_gates, x, z, re, nullptr, n_terms, o, out, state, out_expect)
-                      : run_pauli<W, F64, 2>(ctx, gates, n_gates, x, z, re, im, n_terms, o, out, state, out_expect);
-    if (prec == FFX_FP32)
-        return K == 1 ? run_pauli<W, F32, 1>(ctx, gates, n_gates, x, z, re, nullptr, n_terms, o, out, state, out_expect)
-                      : run_pauli<W, F32, 2>(ctx, gates, n_gates, x, z, re, im, n_terms, o, out, state, out_expect);
-    return K == 1 ? run_pauli<W, F32C, 1>(ctx, gates, n_gates, x, z, re, nullptr, n_terms, o, out, state, out_expect)
-                  : run_pauli<W, F32C, 2>(ctx, gates, n_gates, x, z, re, im, n_terms, o, out, state, out_expect);
+                   const uint64_t* state, double* out_expect, int prec, int K, const ffx_prune_opts* po) {
+#define FFX_RP(CP_, K_) run_pauli<W, CP_, K_>(ctx, gates, n_gates, x, z, re, K_ == 2 ? im : nullptr, n_terms, o, out, \
+                                              state, out_expect, po)
+    if (prec == FFX_FP64) return K == 1 ? FFX_RP(F64, 1) : FFX_RP(F64, 2);
+    if (prec == FFX_FP32) return K == 1 ? FFX_RP(F32, 1) : FFX_RP(F32, 2);
+    return K == 1 ? FFX_RP(F32C, 1) : FFX_RP(F32C, 2);
+#undef FFX_RP
 }
 
 template <int W>
 int dispatch_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, const uint64_t* m,
                       const double* re, const double* im, int64_t n_terms, int maxdegree, double mincoeff,
-                      ffx_terms* out, const uint64_t* occ, double* out_expect, int prec, int K) {
-    if (prec == FFX_FP64)
-        return K == 1 ? run_majorana<W, F64, 1>(ctx, gates, n_gates, m, re, im, n_terms, maxdegree, mincoeff, out, occ,
-                                                out_expect)
-                      : run_majorana<W, F64, 2>(ctx, gates, n_gates, m, re, im, n_terms, maxdegree, mincoeff, out, occ,
-                                                out_expect);
-    if (prec == FFX_FP32)
-        return K == 1 ? run_majorana<W, F32, 1>(ctx, gates, n_gates, m, re, im, n_terms, maxdegree, mincoeff, out, occ,
-                                                out_expect)
-                      : run_majorana<W, F32, 2>(ctx, gates, n_gates, m, re, im, n_terms, maxdegree, mincoeff, out, occ,
-                                                out_expect);
-    return K == 1 ? run_majorana<W, F32C, 1>(ctx, gates, n_gates, m, re, im, n_terms, maxdegree, mincoeff, out, occ,
-                                             out_expect)
-                  : run_majorana<W, F32C, 2>(ctx, gates, n_gates, m, re, im, n_terms, maxdegree, mincoeff, out, occ,
-                                             out_expect);
+                      ffx_terms* out, const uint64_t* occ, double* out_expect, int prec, int K,
+                      const ffx_prune_opts* po) {
+#define FFX_RM(CP_, K_) run_majorana<W, CP_, K_>(ctx, gates, n_gates, m, re, im, n_terms, maxdegree, mincoeff, out, \
+                                                 occ, out_expect, po)
+    if (prec == FFX_FP64) return K == 1 ? FFX_RM(F64, 1) : FFX_RM(F64, 2);
+    if (prec == FFX_FP32) return K == 1 ? FFX_RM(F32, 1) : FFX_RM(F32, 2);
+    return K == 1 ? FFX_RM(F32C, 1) : FFX_RM(F32C, 2);
+#undef FFX_RM
+}
+
+// Term-contribution logs (FP64): forward propagation with one launch per
+// rotation, no compaction, liveness bitmask after every rotation and the
+// coefficients at the logged steps; then the transposed rotations from the last
+// rotation to the first, extracting feature rows and influence targets at the
+// logged steps.
+template <class Alg, int K>
+int run_contrib_log(ffx_ctx* ctx, std::vector<typename Engine<Alg, F64, K>::AStep>& sd, const uint64_t* hx,
+                    const uint64_t* hz, const double* re, const double* im, int64_t n_terms, double mincoeff,
+                    int maxdeg, int max_xw, const ffx_prune_opts& po, const uint64_t* state, int log_every,
+                    int64_t max_rows, float* feats, float* targets, int64_t* n_rows, double* sums, int max_points,
+                    int32_t* n_points) {
+    using Eng = Engine<Alg, F64, K>;
+    using AStep = typename Eng::AStep;
+    for (AStep& a : sd) {
+        if (a.kind != 0) return set_err(ctx, FFX_EINVAL, "contribution logs need a circuit of rotations only");
+        if (a.topk_after) return set_err(ctx, FFX_EINVAL, "contribution logs do not support top-k truncation");
+        a.compact_after = 0;
+    }
+    const size_t N = sd.size();
+    if (log_every < 1) log_every = 1;
+    std::vector<size_t> pts;
+    for (size_t e = 0; e <= N; e += (size_t)log_every) pts.push_back(e);
+    if ((int64_t)pts.size() > (int64_t)max_points) return set_err(ctx, FFX_EINVAL, "max_points is below the number of logged steps");
+    if (max_rows < 0) return set_err(ctx, FFX_EINVAL, "max_rows must be >= 0");
+    const FeatPlan fp(sd, po.rot_per_layer);
+    const PruneGeo geo = prune_geo(po);
+    StateWords sw{};
+    if (state)
+        for (int w = 0; w < FFX_MAX_WORDS; ++w) sw.s[w] = state[w];
+    if (ctx->n_cu <= 0) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+        ctx->n_cu = prop.multiProcessorCount;
+    }
+    size_t want = std::max<size_t>(2 * (size_t)n_terms + 64, (size_t)1 << 16);
+    for (int attempt = 0; attempt < 12; ++attempt) {
+        Eng eng(ctx);
+        CHK(eng.upload(hx, hz, re, im, n_terms));
+        CHK(eng.reserve(want));
+        const size_t cap = ctx->cap, bw = cap / 64;
+        const size_t snap_bytes = pts.size() * (size_t)K * cap * sizeof(double);
+        const size_t bits_bytes = (N + 1) * bw * sizeof(uint64_t);
+        if (snap_bytes + bits_bytes > ((size_t)64 << 30))
+            return set_err(ctx, FFX_ENOMEM, "contribution log history exceeds 64 GiB; raise log_every");
+        uint64_t* bits = nullptr;
+        double *snap = nullptr, *va = nullptr, *vb = nullptr, *dsums = nullptr;
+        float *dfeat = nullptr, *dtarg = nullptr;
+        unsigned long long* counter = nullptr;
+        auto release = [&]() {
+            dev_free(bits);
+            dev_free(snap);
+            dev_free(va);
+            dev_free(vb);
+            dev_free(dsums);
+            dev_free(dfeat);
+            dev_free(dtarg);
+            dev_free(counter);
+        };
+        int rc = dev_alloc(ctx, &bits, bits_bytes);
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &snap, snap_bytes);
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &va, cap * sizeof(double));
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &vb, cap * sizeof(double));
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &dsums, pts.size() * 2 * sizeof(double));
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &dfeat, std::max<size_t>((size_t)max_rows, 1) * kFeat * sizeof(float));
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &dtarg, std::max<size_t>((size_t)max_rows, 1) * sizeof(float));
+        if (rc == FFX_OK) rc = dev_alloc(ctx, &counter, sizeof(unsigned long long));
+        if (rc != FFX_OK) {
+            release();
+            return rc;
+        }
+        auto fail = [&](int code) {
+            release();
+            return code;
+        };
+#define FFX_LCHK(expr)                                                                          \
+    do {                                                                                        \
+        hipError_t _e = (expr);                                                                 \
+        if (_e != hipSuccess) {                                                                 \
+            ctx->err = std::string("HIP error: ") + hipGetErrorString(_e) + " at " #expr;       \
+            return fail(FFX_EHIP);                                                              \
+        }                                                                                       \
+    } while (0)
+        rc = eng.ensure_epochs(N + 1);
+        if (rc != FFX_OK) return fail(rc);
+        const Epochs ep = eng.epochs();
+        FFX_LCHK(hipMemsetAsync(ctx->epochs, 0, Eng::epoch_words(ctx->epochs_cap) * sizeof(uint32_t), ctx->stream));
+        const uint32_t init[2] = {eng.n_host, eng.dead_host};
+        FFX_LCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        FFX_LCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        const unsigned g_bits = grid_for(cap, kBlock, 4096);
+        size_t pi = 0;
+        auto snapshot = [&](size_t e) -> hipError_t {
+            hipLaunchKernelGGL(k_live_bits<F64>, dim3(g_bits), dim3(kBlock), 0, ctx->stream,
+                               (const double*)ctx->coef, ep, (uint32_t)e, (uint32_t)cap, bits + e * bw);
+            if (pi < pts.size() && pts[pi] == e) {
+                hipError_t r = hipMemcpyAsync(snap + pi * K * cap, ctx->coef, (size_t)K * cap * sizeof(double),
+                                              hipMemcpyDeviceToDevice, ctx->stream);
+                ++pi;
+                return r;
+            }
+            return hipGetLastError();
+        };
+        FFX_LCHK(snapshot(0));
+        for (size_t e = 1; e <= N; ++e) {
+            const AStep& a = sd[e - 1];
+            RotParams<F64> rp{};
+            rp.cs = a.cs;
+            rp.sn = a.sn;
+            rp.mincoeff = mincoeff;
+            rp.maxdeg = maxdeg;
+            rp.trunc_thresh = a.trunc_thresh;
+            rp.trunc_xw = a.trunc_xw;
+            rp.max_xw = max_xw;
+            hipLaunchKernelGGL((k_rot_async<Alg, F64, K>), dim3(2 * ctx->n_cu), dim3(kBlockRot), 0, ctx->stream,
+                               eng.store(), (const double*)ctx->coef, (double*)ctx->coef2, a.g, rp, ep, (uint32_t)e);
+            FFX_LCHK(hipGetLastError());
+            std::swap(ctx->coef, ctx->coef2);
+            FFX_LCHK(snapshot(e));
+        }
+        std::vector<uint32_t> hn(N + 1), ha(N + 1);
+        uint32_t hf[2] = {0, 0};
+        FFX_LCHK(hipMemcpyAsync(hn.data(), ep.n, (N + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        FFX_LCHK(hipMemcpyAsync(ha.data(), ep.app, (N + 1) * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        FFX_LCHK(hipMemcpyAsync(hf, ep.flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
+        FFX_LCHK(hipStreamSynchronize(ctx->stream));
+        if (hf[0]) {  // the store overflowed: start again with twice the slots
+            release();
+            want = 2 * cap;
+            continue;
+        }
+        // adjoint sweep
+        const unsigned g_v = grid_for(cap, kBlock, 8192);
+        FFX_LCHK(hipMemsetAsync(dsums, 0, pts.size() * 2 * sizeof(double), ctx->stream));
+        FFX_LCHK(hipMemsetAsync(counter, 0, sizeof(unsigned long long), ctx->stream));
+        hipLaunchKernelGGL(k_adj_init<Alg>, dim3(g_v), dim3(kBlock), 0, ctx->stream, ctx->key, cap, hn[N] + ha[N],
+                           bits + N * bw, sw, va);
+        FFX_LCHK(hipGetLastError());
+        double *cur = va, *nxt = vb;
+        int64_t p = (int64_t)pts.size() - 1;
+        for (size_t e = N + 1; e-- > 0;) {
+            if (p >= 0 && pts[(size_t)p] == e) {
+                hipLaunchKernelGGL((k_log_extract<Alg, K>), dim3(grid_for(hn[e] + ha[e], kBlock, 2048)), dim3(kBlock),
+                                   0, ctx->stream, ctx->key, cap, snap + (size_t)p * K * cap, hn[e] + ha[e],
+                                   bits + e * bw, cur, geo, fp.at(e), (uint64_t)max_rows, counter, dfeat, dtarg,
+                                   dsums + 2 * p);
+                FFX_LCHK(hipGetLastError());
+                --p;
+            }
+            if (e == 0) break;
+            const AStep& a = sd[e - 1];
+            hipLaunchKernelGGL(k_adjoint<Alg>, dim3(grid_for(hn[e], kBlock, 8192)), dim3(kBlock), 0, ctx->stream,
+                               ctx->key, cap, ctx->index, ctx->hslots - 1, a.g, a.cs, a.sn, maxdeg, hn[e],
+                               hn[e] + ha[e], bits + (e - 1) * bw, cur, nxt, &ctx->ctrl->index_full);
+            FFX_LCHK(hipGetLastError());
+            std::swap(cur, nxt);
+        }
+        unsigned long long cnt = 0;
+        FFX_LCHK(hipMemcpyAsync(&cnt, counter, sizeof(cnt), hipMemcpyDeviceToHost, ctx->stream));
+        FFX_LCHK(hipMemcpyAsync(sums, dsums, pts.size() * 2 * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        FFX_LCHK(hipStreamSynchronize(ctx->stream));
+        const size_t rows = std::min<size_t>((size_t)cnt, (size_t)max_rows);
+        if (rows) {
+            if (feats)
+                FFX_LCHK(hipMemcpyAsync(feats, dfeat, rows * kFeat * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            if (targets)
+                FFX_LCHK(hipMemcpyAsync(targets, dtarg, rows * sizeof(float), hipMemcpyDeviceToHost, ctx->stream));
+            FFX_LCHK(hipStreamSynchronize(ctx->stream));
+        }
+#undef FFX_LCHK
+        release();
+        if (n_rows) *n_rows = (int64_t)rows;
+        if (n_points) *n_points = (int32_t)pts.size();
+        CHK(eng.sync_ctrl());
+        ctx->last_kind = 0;
+        return FFX_OK;
+    }
+    return set_err(ctx, FFX_ECAPACITY, "term store kept overflowing");
+}
+
+template <int W>
+int dispatch_log_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const uint64_t* x, const uint64_t* z,
+                       const double* re, const double* im, int64_t n_terms, const ffx_options& o,
+                       const ffx_prune_opts& po, const uint64_t* state, int log_every, int64_t max_rows, float* feats,
+                       float* targets, int64_t* n_rows, double* sums, int max_points, int32_t* n_points, int K) {
+    const std::vector<Step> steps = pauli_schedule<W>(gates, n_gates, o.batched != 0);
+    for (const Step& st : steps)
+        if (st.kind != 0) return set_err(ctx, FFX_EINVAL, "contribution logs need a circuit of rotations only");
+    const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
+    std::vector<CliffGate> cl;
+    if (K == 1) {
+        std::vector<typename Engine<PauliAlg<W>, F64, 1>::AStep> sd;
+        pauli_async_steps<W, F64, 1>(gates, steps, o, sd, cl);
+        return run_contrib_log<PauliAlg<W>, 1>(ctx, sd, x, z, re, nullptr, n_terms, o.mincoeff, maxdeg, o.max_xweight, po,
+                                               state, log_every, max_rows, feats, targets, n_rows, sums, max_points,
+                                               n_points);
+    }
+    std::vector<typename Engine<PauliAlg<W>, F64, 2>::AStep> sd;
+    pauli_async_steps<W, F64, 2>(gates, steps, o, sd, cl);
+    return run_contrib_log<PauliAlg<W>, 2>(ctx, sd, x, z, re, im, n_terms, o.mincoeff, maxdeg, o.max_xweight, po, state,
+                                           log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points);
+}
+
+template <int W, int K>
+void majorana_log_steps(const ffx_majorana_gate* gates, int64_t n_gates, double mincoeff,
+                        std::vector<typename Engine<MajoranaAlg<W>, F64, K>::AStep>& sd) {
+    for (int64_t i = n_gates - 1; i >= 0; --i) {
+        typename Engine<MajoranaAlg<W>, F64, K>::AStep d{};
+        d.g = majorana_gate_dev<W>(gates[i]);
+        gate_trig(gates[i].theta, d.cs, d.sn);
+        d.trunc_thresh = mincoeff > 0 ? 1 : 0;
+        sd.push_back(d);
+    }
+}
+
+template <int W>
+int dispatch_log_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, const uint64_t* m,
+                          const double* re, const double* im, int64_t n_terms, int maxdegree, double mincoeff,
+                          const ffx_prune_opts& po, const uint64_t* occ, int log_every, int64_t max_rows,
+                          float* feats, float* targets, int64_t* n_rows, double* sums, int max_points,
+                          int32_t* n_points, int K) {
+    const int maxdeg = maxdegree < 0 ? (1 << 30) : maxdegree;
+    if (K == 1) {
+        std::vector<typename Engine<MajoranaAlg<W>, F64, 1>::AStep> sd;
+        majorana_log_steps<W, 1>(gates, n_gates, mincoeff, sd);
+        return run_contrib_log<MajoranaAlg<W>, 1>(ctx, sd, m, nullptr, re, im, n_terms, mincoeff, maxdeg, -1, po, occ,
+                                                  log_every, max_rows, feats, targets, n_rows, sums, max_points,
+                                                  n_points);
+    }
+    std::vector<typename Engine<MajoranaAlg<W>, F64, 2>::AStep> sd;
+    majorana_log_steps<W, 2>(gates, n_gates, mincoeff, sd);
+    return run_contrib_log<MajoranaAlg<W>, 2>(ctx, sd, m, nullptr, re, im, n_terms, mincoeff, maxdeg, -1, po, occ,
+                                              log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points);
 }
 
 template <int W>
@@ -1372,10 +1801,18 @@ int dispatch_expect(ffx_ctx* ctx, const uint64_t* state, double* out, int kind, 
 #define FFX_DISPATCH_DECL(EXT, W_)                                                                                  \
     EXT template int dispatch_pauli<W_>(ffx_ctx*, const ffx_pauli_gate*, int64_t, const uint64_t*, const uint64_t*, \
                                         const double*, const double*, int64_t, const ffx_options&, ffx_terms*,     \
-                                        const uint64_t*, double*, int, int);                                       \
+                                        const uint64_t*, double*, int, int, const ffx_prune_opts*);                \
     EXT template int dispatch_majorana<W_>(ffx_ctx*, const ffx_majorana_gate*, int64_t, const uint64_t*,           \
                                            const double*, const double*, int64_t, int, double, ffx_terms*,         \
-                                           const uint64_t*, double*, int, int);                                    \
+                                           const uint64_t*, double*, int, int, const ffx_prune_opts*);             \
+    EXT template int dispatch_log_pauli<W_>(ffx_ctx*, const ffx_pauli_gate*, int64_t, const uint64_t*,            \
+                                            const uint64_t*, const double*, const double*, int64_t,                \
+                                            const ffx_options&, const ffx_prune_opts&, const uint64_t*, int,       \
+                                            int64_t, float*, float*, int64_t*, double*, int, int32_t*, int);       \
+    EXT template int dispatch_log_majorana<W_>(ffx_ctx*, const ffx_majorana_gate*, int64_t, const uint64_t*,       \
+                                               const double*, const double*, int64_t, int, double,                 \
+                                               const ffx_prune_opts&, const uint64_t*, int, int64_t, float*,       \
+                                               float*, int64_t*, double*, int, int32_t*, int);                     \
     EXT template int dispatch_expect<W_>(ffx_ctx*, const uint64_t*, double*, int, int, int);                      \
     EXT template int dispatch_batch<W_>(ffx_ctx*, const ffx_instance*, int32_t, const ffx_options&, int64_t,       \
                                         double*, int64_t*, int32_t*, int, int)

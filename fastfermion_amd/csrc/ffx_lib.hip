@@ -68,6 +68,8 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     dev_free(ctx->epochs);
     dev_free(ctx->topk_buf);
     dev_free(ctx->batch_pool);
+    dev_free(ctx->crot);
+    ffx_mlp_free(ctx);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -105,12 +107,14 @@ int ffx_profile_get(const ffx_ctx* ctx, ffx_profile* out) {
     return FFX_OK;
 }
 
-int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
-                        const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
-                        int64_t n_terms, const ffx_options* opts, ffx_terms* out, const uint64_t* basis_state,
-                        double out_expect[2]) {
-    if (!ctx) return FFX_EINVAL;
-    ctx->err.clear();
+}  // extern "C"
+
+namespace {
+
+// Argument checks shared by the Pauli entry points; K = 2 when any imaginary part is nonzero.
+int check_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words, const uint64_t* x,
+                const uint64_t* z, const double* c_re, const double* c_im, int64_t n_terms, const ffx_options* opts,
+                int* K_out) {
     if (!opts || n_gates < 0 || n_terms < 0 || (n_gates > 0 && !gates) ||
         (n_terms > 0 && (!x || !z || !c_re)))
         return set_err(ctx, FFX_EINVAL, "invalid arguments");
@@ -137,15 +141,151 @@ int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gat
                 K = 2;
                 break;
             }
+    const int prec = opts->precision;
+    if (prec < FFX_FP64 || prec > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
+    *K_out = K;
+    return FFX_OK;
+}
+
+int check_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words, const uint64_t* m,
+                   const double* c_re, const double* c_im, int64_t n_terms, int* K_out) {
+    if (n_gates < 0 || n_terms < 0 || (n_gates > 0 && !gates) || (n_terms > 0 && (!m || !c_re)))
+        return set_err(ctx, FFX_EINVAL, "invalid arguments");
+    if (n_words < 1 || n_words > FFX_MAX_WORDS) return set_err(ctx, FFX_EINVAL, "n_words must be in 1..4");
+    if (n_terms > 0x3FFFFFFF) return set_err(ctx, FFX_ECAPACITY, "too many input terms");
+    for (int64_t i = 0; i < n_gates; ++i)
+        for (int w = n_words; w < FFX_MAX_WORDS; ++w)
+            if (gates[i].m[w]) return set_err(ctx, FFX_EINVAL, "rotation axis exceeds n_words");
+    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, FFX_ENODEV, "hipSetDevice failed");
+    // normalized coefficients d = c / i^r are real for every term iff K = 1 suffices
+    int K = 1;
+    for (int64_t i = 0; i < n_terms && K == 1; ++i) {
+        const int r = mono_degree_mod4_r(m + (size_t)i * n_words, n_words);
+        const double dim = r ? -c_re[i] : (c_im ? c_im[i] : 0.0);
+        if (dim != 0.0) K = 2;
+    }
+    *K_out = K;
+    return FFX_OK;
+}
+
+int check_prune(ffx_ctx* ctx, const ffx_prune_opts* po) {
+    if (!po) return FFX_OK;
+    if (!(po->clifford_tol >= 0.0) || po->clifford_tol >= 1.0)
+        return set_err(ctx, FFX_EINVAL, "clifford_tol must be in [0, 1)");
+    if (po->prune_period < 0) return set_err(ctx, FFX_EINVAL, "prune_period must be >= 0");
+    if (po->obs_site < 0 || po->obs_site > 255) return set_err(ctx, FFX_EINVAL, "obs_site must be in 0..255");
+    return FFX_OK;
+}
+
+int pauli_propagate_impl(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                         const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im, int64_t n_terms,
+                         const ffx_options* opts, const ffx_prune_opts* po, ffx_terms* out,
+                         const uint64_t* basis_state, double out_expect[2]) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    int K = 1;
+    CHK(check_pauli(ctx, gates, n_gates, n_words, x, z, c_re, c_im, n_terms, opts, &K));
+    CHK(check_prune(ctx, po));
     ctx->prof = ffx_profile{};
     ctx->launches = 0;
     const int prec = opts->precision;
-    if (prec < FFX_FP64 || prec > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
     switch (n_words) {
-        case 1: return dispatch_pauli<1>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
-        case 2: return dispatch_pauli<2>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
-        case 3: return dispatch_pauli<3>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
-        default: return dispatch_pauli<4>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K);
+        case 1: return dispatch_pauli<1>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K, po);
+        case 2: return dispatch_pauli<2>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K, po);
+        case 3: return dispatch_pauli<3>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K, po);
+        default: return dispatch_pauli<4>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, out, basis_state, out_expect, prec, K, po);
+    }
+}
+
+int majorana_propagate_impl(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,
+                            const uint64_t* m, const double* c_re, const double* c_im, int64_t n_terms,
+                            int32_t maxdegree, double mincoeff, int32_t precision, const ffx_prune_opts* po,
+                            ffx_terms* out, const uint64_t* occ, double out_expect[2]) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    int K = 1;
+    CHK(check_majorana(ctx, gates, n_gates, n_words, m, c_re, c_im, n_terms, &K));
+    CHK(check_prune(ctx, po));
+    ctx->prof = ffx_profile{};
+    ctx->launches = 0;
+    if (precision < FFX_FP64 || precision > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
+    switch (n_words) {
+        case 1: return dispatch_majorana<1>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K, po);
+        case 2: return dispatch_majorana<2>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K, po);
+        case 3: return dispatch_majorana<3>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K, po);
+        default: return dispatch_majorana<4>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K, po);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                        const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
+                        int64_t n_terms, const ffx_options* opts, ffx_terms* out, const uint64_t* basis_state,
+                        double out_expect[2]) {
+    return pauli_propagate_impl(ctx, gates, n_gates, n_words, x, z, c_re, c_im, n_terms, opts, nullptr, out,
+                                basis_state, out_expect);
+}
+
+int ffx_pauli_propagate_pruned(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                               const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
+                               int64_t n_terms, const ffx_options* opts, const ffx_prune_opts* po,
+                               ffx_terms* out, const uint64_t* basis_state, double out_expect[2]) {
+    return pauli_propagate_impl(ctx, gates, n_gates, n_words, x, z, c_re, c_im, n_terms, opts, po, out, basis_state,
+                                out_expect);
+}
+
+int ffx_majorana_propagate_pruned(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,
+                                  const uint64_t* m, const double* c_re, const double* c_im, int64_t n_terms,
+                                  int32_t maxdegree, double mincoeff, int32_t precision, const ffx_prune_opts* po,
+                                  ffx_terms* out, const uint64_t* occ, double out_expect[2]) {
+    return majorana_propagate_impl(ctx, gates, n_gates, n_words, m, c_re, c_im, n_terms, maxdegree, mincoeff,
+                                   precision, po, out, occ, out_expect);
+}
+
+int ffx_pauli_contrib_log(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                          const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
+                          int64_t n_terms, const ffx_options* opts, const ffx_prune_opts* po,
+                          const uint64_t* basis_state, int32_t log_every, int64_t max_rows, float* feats,
+                          float* targets, int64_t* n_rows, double* sums, int32_t max_points, int32_t* n_points) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    if (!po || !sums || max_rows < 0 || (max_rows > 0 && (!feats || !targets)))
+        return set_err(ctx, FFX_EINVAL, "invalid arguments");
+    int K = 1;
+    CHK(check_pauli(ctx, gates, n_gates, n_words, x, z, c_re, c_im, n_terms, opts, &K));
+    CHK(check_prune(ctx, po));
+    ctx->prof = ffx_profile{};
+    ctx->launches = 0;
+    switch (n_words) {
+        case 1: return dispatch_log_pauli<1>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, *po, basis_state, log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points, K);
+        case 2: return dispatch_log_pauli<2>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, *po, basis_state, log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points, K);
+        case 3: return dispatch_log_pauli<3>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, *po, basis_state, log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points, K);
+        default: return dispatch_log_pauli<4>(ctx, gates, n_gates, x, z, c_re, c_im, n_terms, *opts, *po, basis_state, log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points, K);
+    }
+}
+
+int ffx_majorana_contrib_log(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,
+                             const uint64_t* m, const double* c_re, const double* c_im, int64_t n_terms,
+                             int32_t maxdegree, double mincoeff, const ffx_prune_opts* po, const uint64_t* occ,
+                             int32_t log_every, int64_t max_rows, float* feats, float* targets, int64_t* n_rows,
+                             double* sums, int32_t max_points, int32_t* n_points) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    if (!po || !sums || max_rows < 0 || (max_rows > 0 && (!feats || !targets)))
+        return set_err(ctx, FFX_EINVAL, "invalid arguments");
+    int K = 1;
+    CHK(check_majorana(ctx, gates, n_gates, n_words, m, c_re, c_im, n_terms, &K));
+    CHK(check_prune(ctx, po));
+    ctx->prof = ffx_profile{};
+    ctx->launches = 0;
+    switch (n_words) {
+        case 1: return dispatch_log_majorana<1>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, *po, occ, log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points, K);
+        case 2: return dispatch_log_majorana<2>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, *po, occ, log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points, K);
+        case 3: return dispatch_log_majorana<3>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, *po, occ, log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points, K);
+        default: return dispatch_log_majorana<4>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, *po, occ, log_every, max_rows, feats, targets, n_rows, sums, max_points, n_points, K);
     }
 }
 
@@ -200,32 +340,8 @@ int ffx_majorana_propagate(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t
                            const uint64_t* m, const double* c_re, const double* c_im, int64_t n_terms,
                            int32_t maxdegree, double mincoeff, int32_t precision, ffx_terms* out,
                            const uint64_t* occ, double out_expect[2]) {
-    if (!ctx) return FFX_EINVAL;
-    ctx->err.clear();
-    if (n_gates < 0 || n_terms < 0 || (n_gates > 0 && !gates) || (n_terms > 0 && (!m || !c_re)))
-        return set_err(ctx, FFX_EINVAL, "invalid arguments");
-    if (n_words < 1 || n_words > FFX_MAX_WORDS) return set_err(ctx, FFX_EINVAL, "n_words must be in 1..4");
-    if (n_terms > 0x3FFFFFFF) return set_err(ctx, FFX_ECAPACITY, "too many input terms");
-    for (int64_t i = 0; i < n_gates; ++i)
-        for (int w = n_words; w < FFX_MAX_WORDS; ++w)
-            if (gates[i].m[w]) return set_err(ctx, FFX_EINVAL, "rotation axis exceeds n_words");
-    if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, FFX_ENODEV, "hipSetDevice failed");
-    // normalized coefficients d = c / i^r are real for every term iff K = 1 suffices
-    int K = 1;
-    for (int64_t i = 0; i < n_terms && K == 1; ++i) {
-        const int r = mono_degree_mod4_r(m + (size_t)i * n_words, n_words);
-        const double dim = r ? -c_re[i] : (c_im ? c_im[i] : 0.0);
-        if (dim != 0.0) K = 2;
-    }
-    ctx->prof = ffx_profile{};
-    ctx->launches = 0;
-    if (precision < FFX_FP64 || precision > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
-    switch (n_words) {
-        case 1: return dispatch_majorana<1>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K);
-        case 2: return dispatch_majorana<2>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K);
-        case 3: return dispatch_majorana<3>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K);
-        default: return dispatch_majorana<4>(ctx, gates, n_gates, m, c_re, c_im, n_terms, maxdegree, mincoeff, out, occ, out_expect, precision, K);
-    }
+    return majorana_propagate_impl(ctx, gates, n_gates, n_words, m, c_re, c_im, n_terms, maxdegree, mincoeff,
+                                   precision, nullptr, out, occ, out_expect);
 }
 
 int ffx_expect_last(ffx_ctx* ctx, const uint64_t* state, double out[2]) {

@@ -197,3 +197,21 @@ def fermi_hubbard(lx: int = 6, ly: int = 6, t: float = 1.0, u: float = 4.0, dt: 
     m[1, 0] = np.uint64(0b11)
     c = np.array([0.5, 0.5j])
     return gates, (m, c)
+
+
+def lattice_geometry(L: int):
+    """Rows and columns of the qubits of an L x L lattice (site s = L*r + c)."""
+    return [s // L for s in range(L * L)], [s % L for s in range(L * L)]
+
+
+def hubbard_geometry(lx: int, ly: int):
+    """Rows and columns of every Majorana index of fermi_hubbard_terms: Majorana
+    m belongs to mode m // 2, site m // 4 = x + lx*y."""
+    n = 4 * lx * ly
+    return [(m // 4) // lx for m in range(n)], [(m // 4) % lx for m in range(n)]
+
+
+def kicked_ising_family(L: int = 10, layers: int = 30, theta_lo: float = 0.3, theta_hi: float = 0.7, seed: int = 7):
+    """Kicked-Ising circuit of the learned-pruning workload: RZZ(-pi/2) on every
+    bond, then RX(theta_s) with theta_s uniform in [theta_lo, theta_hi] (seeded)."""
+    return kicked_ising(L, layers, theta_lo=theta_lo, theta_hi=theta_hi, seed=seed)

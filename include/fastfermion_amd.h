@@ -198,6 +198,92 @@ int ffx_set_kernel_timing(ffx_ctx* ctx, int every);
 /* Number of visible HIP devices (0 when none); never fails. */
 int ffx_device_count(void);
 
+/* ------------------------------------------------------------------------
+ * Pruned propagation and the learned term pruner (no reference counterpart:
+ * the reference truncates by degree, |c|, top-k and X/Y weight only,
+ * src/pauli/truncate.h:16-57).  Both change results; they are opt-in and the
+ * entry points above never use them.
+ * ------------------------------------------------------------------------ */
+
+#define FFX_PRUNER_FEATURES 64
+#define FFX_PRUNER_HIDDEN 128
+/* MLP 64 -> 128 -> 128 -> 1 (ReLU).  Flat fp32 parameter vector:
+ * W1[64][128] (in x out), b1[128], W2[128][128], b2[128], w3[128], b3[1]. */
+#define FFX_PRUNER_NPARAMS 24961
+
+/* Options of the pruned mode and the site geometry used by the term features. */
+typedef struct ffx_prune_opts {
+    double clifford_tol;    /* rotations with |cos theta| <= clifford_tol keep only their sin branch and
+                               run as fused Clifford maps (a run of them is one kernel); 0: off */
+    float log10_tau;        /* erase terms with log10|c| + predicted log10|influence| < log10_tau */
+    int32_t prune_period;   /* run the learned pruner after every prune_period-th rotation; 0: off */
+    int32_t rot_per_layer;  /* rotations per circuit layer (feature: remaining layers) */
+    int32_t obs_site;       /* qubit / Majorana index of the observable's site (feature: distance) */
+    uint8_t row[256];       /* lattice row and column of every qubit (Pauli) or Majorana index */
+    uint8_t col[256];
+} ffx_prune_opts;
+
+/* ffx_pauli_propagate with the pruned mode of `po` (NULL: identical to ffx_pauli_propagate).
+ * The learned pruner needs real coefficients and a model (ffx_pruner_init / _set_params). */
+int ffx_pauli_propagate_pruned(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                               const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
+                               int64_t n_terms, const ffx_options* opts, const ffx_prune_opts* po,
+                               ffx_terms* out, const uint64_t* basis_state, double out_expect[2]);
+
+/* ffx_majorana_propagate with the pruned mode of `po`. */
+int ffx_majorana_propagate_pruned(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,
+                                  const uint64_t* m, const double* c_re, const double* c_im, int64_t n_terms,
+                                  int32_t maxdegree, double mincoeff, int32_t precision, const ffx_prune_opts* po,
+                                  ffx_terms* out, const uint64_t* occ, double out_expect[2]);
+
+/* Term-contribution logs: one plain FP64 propagation (one launch per rotation,
+ * reference schedule and truncation of `opts`; no Clifford blocks, no top-k),
+ * then the transposed rotations in the opposite order give every live term q
+ * after rotation e its influence v_q(e) on the final expectation <b|O|b>
+ * (b = basis_state / occ; NULL: all zeros).  At e = 0, log_every, 2*log_every,
+ * ... <= n_rotations, every live term yields one row: FFX_PRUNER_FEATURES
+ * features (same as the pruner uses; geometry from `po`) into feats and
+ * clamp(log10(|v| + 1e-12), -12, 0)/4 into targets, at most max_rows rows in
+ * total.  sums[2p], sums[2p+1] receive sum_q c_q v_q (re, im) at the p-th
+ * logged step (n_points of them), each equal to the final expectation up to
+ * rounding.  *n_rows: rows written. */
+int ffx_pauli_contrib_log(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                          const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
+                          int64_t n_terms, const ffx_options* opts, const ffx_prune_opts* po,
+                          const uint64_t* basis_state, int32_t log_every, int64_t max_rows, float* feats,
+                          float* targets, int64_t* n_rows, double* sums, int32_t max_points, int32_t* n_points);
+
+int ffx_majorana_contrib_log(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,
+                             const uint64_t* m, const double* c_re, const double* c_im, int64_t n_terms,
+                             int32_t maxdegree, double mincoeff, const ffx_prune_opts* po, const uint64_t* occ,
+                             int32_t log_every, int64_t max_rows, float* feats, float* targets, int64_t* n_rows,
+                             double* sums, int32_t max_points, int32_t* n_points);
+
+/* The context's pruner model (one per context).  init: He-initialised weights
+ * from a SplitMix64 seed; set/get: the flat fp32 parameter vector (n must be
+ * FFX_PRUNER_NPARAMS).  The device keeps bf16 copies of the weights for its
+ * MFMA kernels and the fp32 vector as the master copy for training. */
+int ffx_pruner_init(ffx_ctx* ctx, uint64_t seed);
+int ffx_pruner_set_params(ffx_ctx* ctx, const float* params, int64_t n);
+int ffx_pruner_get_params(ffx_ctx* ctx, float* params, int64_t n);
+
+/* Forward pass on n feature rows (n x FFX_PRUNER_FEATURES floats, rounded to
+ * bf16 on upload): out[i] = predicted target of row i. */
+int ffx_pruner_forward(ffx_ctx* ctx, const float* feats, int64_t n, float* out);
+
+/* Loss (1/n) sum_i w_i (y_i - t_i)^2 over n rows, w_i = under_weight when the
+ * prediction y_i is below the target t_i and 1 otherwise (under_weight = 1: mean
+ * squared error), and its gradient with respect to every parameter (the bf16
+ * weights' gradient, applied to the fp32 master copy). */
+int ffx_pruner_loss_grad(ffx_ctx* ctx, const float* feats, const float* targets, int64_t n, float under_weight,
+                         double* loss, float* grad);
+
+/* Adam training on the device with the loss of ffx_pruner_loss_grad: `epochs`
+ * passes over n rows in shuffled minibatches of `batch` rows (seeded), learning
+ * rate lr.  epoch_loss[e] (optional) receives the loss of epoch e during training. */
+int ffx_pruner_train(ffx_ctx* ctx, const float* feats, const float* targets, int64_t n, int32_t epochs,
+                     int32_t batch, float lr, float under_weight, uint64_t seed, double* epoch_loss);
+
 #ifdef __cplusplus
 }
 #endif
