@@ -26,7 +26,8 @@ EXPORTED = ("ffx_ctx_create", "ffx_ctx_destroy", "ffx_last_error", "ffx_pauli_pr
             "ffx_device_count", "ffx_set_kernel_timing", "ffx_pauli_propagate_batch",
             "ffx_pauli_propagate_pruned", "ffx_majorana_propagate_pruned", "ffx_pauli_contrib_log",
             "ffx_majorana_contrib_log", "ffx_pruner_init", "ffx_pruner_set_params", "ffx_pruner_get_params",
-            "ffx_pruner_forward", "ffx_pruner_loss_grad", "ffx_pruner_train")
+            "ffx_pruner_forward", "ffx_pruner_loss_grad", "ffx_pruner_train", "ffx_batch_autotune",
+            "ffx_batch_graph_replays")
 
 PRUNER_FEATURES = 64
 PRUNER_NPARAMS = 24961
@@ -70,6 +71,11 @@ class Instance(ctypes.Structure):
     _fields_ = [("gates", ctypes.c_void_p), ("n_gates", ctypes.c_int64), ("x", ctypes.c_void_p),
                 ("z", ctypes.c_void_p), ("c_re", ctypes.c_void_p), ("c_im", ctypes.c_void_p),
                 ("n_terms", ctypes.c_int64), ("basis_state", ctypes.c_uint64 * 4)]
+
+
+class BatchTuning(ctypes.Structure):
+    _fields_ = [("block", ctypes.c_int32), ("window", ctypes.c_int32), ("slots", ctypes.c_int64),
+                ("ms", ctypes.c_double), ("candidates", ctypes.c_int32), ("sample", ctypes.c_int32)]
 
 
 class Profile(ctypes.Structure):
@@ -119,6 +125,9 @@ def load():
                                                 i32, i64, vp, vp, vp, vp, i32, vp]
             L.ffx_majorana_contrib_log.argtypes = [vp, vp, i64, i32, vp, vp, vp, i64, i32, dbl, PO, vp, i32, i64, vp,
                                                    vp, vp, vp, i32, vp]
+            L.ffx_batch_autotune.argtypes = [vp, ctypes.POINTER(Instance), i32, i32, ctypes.POINTER(Options), i32,
+                                             ctypes.POINTER(BatchTuning)]
+            L.ffx_batch_graph_replays.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
             L.ffx_pruner_init.argtypes = [vp, ctypes.c_uint64]
             L.ffx_pruner_set_params.argtypes = [vp, vp, i64]
             L.ffx_pruner_get_params.argtypes = [vp, vp, i64]
@@ -226,6 +235,36 @@ class Context:
         up to 4 uint64 words (or None for |0...0>).  Returns (expectations
         complex[n], live term counts int64[n], number of instances that ran on
         the device-wide engine)."""
+        arr, keep = self._instances(instances, n_words)
+        n = len(instances)
+        opts = Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        ex = np.zeros(2 * max(n, 1))
+        nt = np.zeros(max(n, 1), np.int64)
+        fb = ctypes.c_int32(0)
+        self._check(self._L.ffx_pauli_propagate_batch(self._h, arr, n, n_words, ctypes.byref(opts), slots, _p(ex),
+                                                      _p(nt), ctypes.byref(fb)))
+        del keep
+        return ex[0:2 * n:2] + 1j * ex[1:2 * n:2], nt[:n], int(fb.value)
+
+    def batch_autotune(self, instances, n_words, sample=128, maxdegree=-1, mincoeff=0.0, topk=0, max_xweight=-1,
+                       xtrunc_period=1, batched=True, precision=FP64):
+        """Autotune the batched launch on the longest `sample` instances; later batches
+        with slots=0 use the result.  Returns the chosen configuration as a dict."""
+        arr, keep = self._instances(instances, n_words)
+        opts = Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        t = BatchTuning()
+        self._check(self._L.ffx_batch_autotune(self._h, arr, len(instances), n_words, ctypes.byref(opts), sample,
+                                               ctypes.byref(t)))
+        del keep
+        return {k: getattr(t, k) for k, _ in BatchTuning._fields_}
+
+    def batch_graph_replays(self) -> int:
+        v = ctypes.c_int64(0)
+        self._check(self._L.ffx_batch_graph_replays(self._h, ctypes.byref(v)))
+        return int(v.value)
+
+    @staticmethod
+    def _instances(instances, n_words):
         keep = []
         arr = (Instance * max(len(instances), 1))()
         for i, (gates, x, z, c, state) in enumerate(instances):
@@ -241,14 +280,7 @@ class Context:
             arr[i].n_terms = len(c)
             for w, v in enumerate(list(state or [])[:4]):
                 arr[i].basis_state[w] = int(v)
-        n = len(instances)
-        opts = Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
-        ex = np.zeros(2 * max(n, 1))
-        nt = np.zeros(max(n, 1), np.int64)
-        fb = ctypes.c_int32(0)
-        self._check(self._L.ffx_pauli_propagate_batch(self._h, arr, n, n_words, ctypes.byref(opts), slots, _p(ex),
-                                                      _p(nt), ctypes.byref(fb)))
-        return ex[0:2 * n:2] + 1j * ex[1:2 * n:2], nt[:n], int(fb.value)
+        return arr, keep
 
     def majorana_propagate(self, gates, n_words, m, c, maxdegree=-1, mincoeff=0.0, precision=FP64,
                            download=True, occ=None, want_expect=False):

@@ -96,6 +96,18 @@ struct ffx_ctx {
     ffx_profile prof{};
     int64_t launches = 0;
     MlpState* mlp = nullptr;  // pruner model (ffx_pruner_*)
+    // batched propagation: autotuned configurations, pinned host image, captured graph
+    struct BatchCfg {
+        int W = 0, prec = 0, K = 0;
+        int block = 0, window = 0;
+        int64_t slots = 0;
+    };
+    std::vector<BatchCfg> batch_tuned;
+    void* batch_host = nullptr;  // pinned
+    size_t batch_host_bytes = 0;
+    hipGraphExec_t batch_exec = nullptr;
+    size_t batch_key[8] = {0, 0, 0, 0, 0, 0, 0, 0};  // layout the captured graph was built for
+    int64_t batch_graph_launches = 0;  // replays of a captured graph (profile)
     void* crot = nullptr;     // Clifford-rotation records of the pruned mode
     size_t crot_bytes = 0;
 };
@@ -342,7 +354,12 @@ struct Engine {
             return FFX_OK;
         if (need > 0x7FFFFFF0ull) return set_err(ctx, FFX_ECAPACITY, "term count exceeds 2^31 slots");
         size_t ncap = pow2ceil(std::max<size_t>(need, 1024));
-        if (ncap < 2 * ctx->cap && ctx->key_planes == NP) ncap = std::max<size_t>(ncap, 2 * ctx->cap);
+        const bool same_layout =
+            ctx->key_planes == NP && ctx->coef_planes == coef_planes() && ctx->coef_elem == sizeof(store_t);
+        // growth of the same layout doubles (amortised overflow restarts); a layout change
+        // (key width, precision, real/complex) keeps the capacity the context already had
+        if (same_layout && ncap < 2 * ctx->cap) ncap = std::max<size_t>(ncap, 2 * ctx->cap);
+        if (!same_layout) ncap = std::max<size_t>(ncap, ctx->cap);
         uint64_t *nkey = nullptr, *nkey2 = nullptr, *nindex = nullptr;
         void *ncoef = nullptr, *ncoef2 = nullptr;
         const size_t cbytes = coef_planes() * ncap * sizeof(store_t);
@@ -1314,9 +1331,17 @@ inline int mono_degree_mod4_r(const uint64_t* m, int W) {
 
 // Batched propagation: one workgroup per instance (k_batch_run); instances that
 // are not eligible or outgrow their slot budget run on the device-wide engine.
+// Configuration of a batched launch: threads per instance workgroup, maximum
+// window length, slot budget per instance (0 in a field: the tuned or default value).
+struct BatchConf {
+    int block = 0;
+    int window = 0;
+    int64_t slots = 0;
+};
+
 template <int W, class CP, int K>
 int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, const ffx_options& o, int64_t slots,
-                    double* out_expect, int64_t* out_terms, int32_t* out_fallback) {
+                    double* out_expect, int64_t* out_terms, int32_t* out_fallback, const BatchConf* conf = nullptr) {
     using Alg = PauliAlg<W>;
     using Eng = Engine<Alg, CP, K>;
     using AStep = typename Eng::AStep;
@@ -1324,9 +1349,27 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     constexpr int NP = Alg::NP;
     const double t0 = now_s();
     const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
+    // configuration: explicit > autotuned for this (W, precision, K) > defaults
+    const int prec_id = std::is_same<CP, F64>::value ? FFX_FP64 : (std::is_same<CP, F32>::value ? FFX_FP32 : FFX_FP32C);
+    BatchConf cf{kBlockBatch, kWin, 32768};
+    for (const auto& t : ctx->batch_tuned)
+        if (t.W == W && t.prec == prec_id && t.K == K) cf = BatchConf{t.block, t.window, t.slots};
+    if (conf) {
+        if (conf->block) cf.block = conf->block;
+        if (conf->window) cf.window = conf->window;
+        if (conf->slots) cf.slots = conf->slots;
+    }
+    if (slots > 0) cf.slots = slots;
+    if (const char* ev = getenv("FFX_BATCH_BLOCK")) {
+        const int v = atoi(ev);
+        if (v == 64 || v == 128 || v == 256 || v == 512) cf.block = v;
+    }
+    if (cf.block != 64 && cf.block != 128 && cf.block != 256 && cf.block != 512)
+        return set_err(ctx, FFX_EINVAL, "batch workgroup size must be 64, 128, 256 or 512");
+    cf.window = std::max(1, std::min(kWin, cf.window));
     int64_t max_terms = 1;
     for (int32_t i = 0; i < n_inst; ++i) max_terms = std::max(max_terms, inst[i].n_terms);
-    const size_t S = pow2ceil((size_t)std::max<int64_t>(std::max<int64_t>(slots > 0 ? slots : 32768, 2 * max_terms), 1024));
+    const size_t S = pow2ceil((size_t)std::max<int64_t>(std::max<int64_t>(cf.slots, 2 * max_terms), 1024));
     if (S > 0x40000000ull) return set_err(ctx, FFX_ECAPACITY, "slot budget too large");
     // schedules: eligible instances get their windows, the others the device-wide engine
     std::vector<std::vector<BStep<Alg, CP>>> sched(n_inst);
@@ -1340,7 +1383,7 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
         std::vector<CliffGate> cl;
         pauli_async_steps<W, CP, K>(inst[i].gates, steps, o, sd, cl);
         for (size_t t = 0; t < sd.size();) {
-            const size_t kw = Eng::window_len(sd, t, kWin);
+            const size_t kw = Eng::window_len(sd, t, cf.window);
             BStep<Alg, CP> b{};
             b.w = Eng::make_window(sd, t, kw, o.mincoeff, maxdeg, o.max_xweight);
             sched[i].push_back(b);
@@ -1392,19 +1435,33 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             ctx->batch_pool_bytes = total;
         }
         char* pool = (char*)ctx->batch_pool;
-        std::vector<InstState<CP>> st(ne);
-        std::vector<BStep<Alg, CP>> hsteps;
-        hsteps.reserve(n_steps);
-        std::vector<uint64_t> hx, hz;
-        std::vector<double> hre, him;
-        std::vector<int64_t> hoff(ne + 1, 0);
-        hx.reserve(n_in * W);
-        hz.reserve(n_in * W);
+        // host image of pool[off_states, total) in pinned memory: states, schedules, inputs, offsets
+        const size_t img = total - off_states;
+        if (img > ctx->batch_host_bytes) {
+            if (ctx->batch_exec) {  // the captured graph copies from the old buffer
+                (void)hipGraphExecDestroy(ctx->batch_exec);
+                ctx->batch_exec = nullptr;
+            }
+            if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
+            ctx->batch_host = nullptr;
+            ctx->batch_host_bytes = 0;
+            HIPCHK(hipHostMalloc(&ctx->batch_host, img, hipHostMallocDefault));
+            ctx->batch_host_bytes = img;
+        }
+        char* H = (char*)ctx->batch_host;
+        InstState<CP>* hst = (InstState<CP>*)H;
+        BStep<Alg, CP>* hsteps = (BStep<Alg, CP>*)(H + (off_steps - off_states));
+        uint64_t* hx = (uint64_t*)(H + (off_x - off_states));
+        uint64_t* hz = (uint64_t*)(H + (off_z - off_states));
+        double* hre = (double*)(H + (off_re - off_states));
+        double* him = (double*)(H + (off_im - off_states));
+        int64_t* hoff = (int64_t*)(H + (off_off - off_states));
+        size_t ns_done = 0;
+        hoff[0] = 0;
         for (size_t b = 0; b < ne; ++b) {
             const ffx_instance& in = inst[elig[b]];
             char* base = pool + b * store_bytes;
-            InstState<CP>& s = st[b];
-            s = InstState<CP>{};
+            InstState<CP> s = InstState<CP>{};
             s.key = (uint64_t*)base;
             s.key2 = s.key + NP * S;
             s.coef = (store_t*)(s.key2 + NP * S);
@@ -1413,49 +1470,73 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             s.hmask = hs - 1;
             s.cap = (uint32_t)S;
             s.n = (uint32_t)in.n_terms;
-            s.steps = (const void*)(pool + off_steps + hsteps.size() * sizeof(BStep<Alg, CP>));
+            s.steps = (const void*)(pool + off_steps + ns_done * sizeof(BStep<Alg, CP>));
             s.n_steps = (uint32_t)sched[elig[b]].size();
             for (int w = 0; w < FFX_MAX_WORDS_DEV; ++w) s.state[w] = in.basis_state[w];
-            hsteps.insert(hsteps.end(), sched[elig[b]].begin(), sched[elig[b]].end());
+            hst[b] = s;
+            std::memcpy(hsteps + ns_done, sched[elig[b]].data(), sched[elig[b]].size() * sizeof(BStep<Alg, CP>));
+            ns_done += sched[elig[b]].size();
+            const int64_t o0 = hoff[b];
             for (int64_t t = 0; t < in.n_terms; ++t) {
                 for (int w = 0; w < W; ++w) {
-                    hx.push_back(in.x[t * W + w]);
-                    hz.push_back(in.z[t * W + w]);
+                    hx[(size_t)(o0 + t) * W + w] = in.x[t * W + w];
+                    hz[(size_t)(o0 + t) * W + w] = in.z[t * W + w];
                 }
-                hre.push_back(in.c_re[t]);
-                him.push_back(in.c_im ? in.c_im[t] : 0.0);
+                hre[o0 + t] = in.c_re[t];
+                him[o0 + t] = in.c_im ? in.c_im[t] : 0.0;
             }
-            hoff[b + 1] = hoff[b] + in.n_terms;
+            hoff[b + 1] = o0 + in.n_terms;
         }
         InstState<CP>* dst = (InstState<CP>*)(pool + off_states);
-        HIPCHK(hipMemcpyAsync(dst, st.data(), ne * sizeof(InstState<CP>), hipMemcpyHostToDevice, ctx->stream));
-        if (!hsteps.empty())
-            HIPCHK(hipMemcpyAsync(pool + off_steps, hsteps.data(), hsteps.size() * sizeof(BStep<Alg, CP>),
-                                  hipMemcpyHostToDevice, ctx->stream));
-        if (n_in) {
-            HIPCHK(hipMemcpyAsync(pool + off_x, hx.data(), hx.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(pool + off_z, hz.data(), hz.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(pool + off_re, hre.data(), hre.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-            HIPCHK(hipMemcpyAsync(pool + off_im, him.data(), him.size() * 8, hipMemcpyHostToDevice, ctx->stream));
+        const unsigned bb = (unsigned)cf.block;
+        // one captured graph per pool layout: upload, unpack, run, download of the states.
+        // Calls with the same layout (angle sweeps, repeated batches) replay it.
+        const size_t key[8] = {total, off_states, ne, (size_t)bb, (size_t)W, (size_t)prec_id, (size_t)K,
+                               (size_t)(uintptr_t)pool};
+        auto enqueue = [&]() -> hipError_t {
+            hipError_t r = hipMemcpyAsync(pool + off_states, H, img, hipMemcpyHostToDevice, ctx->stream);
+            if (r != hipSuccess) return r;
+            hipLaunchKernelGGL((k_batch_unpack<W, CP, K>), dim3((unsigned)ne), dim3(kBlock), 0, ctx->stream, dst,
+                               (const uint64_t*)(pool + off_x), (const uint64_t*)(pool + off_z),
+                               (const double*)(pool + off_re), (const double*)(pool + off_im),
+                               (const int64_t*)(pool + off_off));
+            hipLaunchKernelGGL((k_batch_run<Alg, CP, K>), dim3((unsigned)ne), dim3(bb), 0, ctx->stream, dst);
+            r = hipGetLastError();
+            if (r != hipSuccess) return r;
+            return hipMemcpyAsync(H, dst, ne * sizeof(InstState<CP>), hipMemcpyDeviceToHost, ctx->stream);
+        };
+        bool have = ctx->batch_exec && std::memcmp(key, ctx->batch_key, sizeof(key)) == 0;
+        if (!have && !getenv("FFX_BATCH_NO_GRAPH")) {
+            if (ctx->batch_exec) {
+                (void)hipGraphExecDestroy(ctx->batch_exec);
+                ctx->batch_exec = nullptr;
+            }
+            hipGraph_t graph = nullptr;
+            if (hipStreamBeginCapture(ctx->stream, hipStreamCaptureModeThreadLocal) == hipSuccess) {
+                const hipError_t er = enqueue();
+                const hipError_t ec = hipStreamEndCapture(ctx->stream, &graph);
+                if (er == hipSuccess && ec == hipSuccess && graph &&
+                    hipGraphInstantiate(&ctx->batch_exec, graph, nullptr, nullptr, 0) == hipSuccess) {
+                    std::memcpy(ctx->batch_key, key, sizeof(key));
+                    have = true;
+                } else {
+                    ctx->batch_exec = nullptr;
+                }
+                if (graph) (void)hipGraphDestroy(graph);
+            }
+            (void)hipGetLastError();  // a failed capture falls back to direct launches
         }
-        HIPCHK(hipMemcpyAsync(pool + off_off, hoff.data(), hoff.size() * 8, hipMemcpyHostToDevice, ctx->stream));
-        hipLaunchKernelGGL((k_batch_unpack<W, CP, K>), dim3((unsigned)ne), dim3(kBlock), 0, ctx->stream, dst,
-                           (const uint64_t*)(pool + off_x), (const uint64_t*)(pool + off_z),
-                           (const double*)(pool + off_re), (const double*)(pool + off_im),
-                           (const int64_t*)(pool + off_off));
-        HIPCHK(hipGetLastError());
         if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->ev0, ctx->stream));
-        unsigned bb = kBlockBatch;
-        if (const char* ev = getenv("FFX_BATCH_BLOCK")) {
-            const int v = atoi(ev);
-            if (v == 64 || v == 128 || v == 256 || v == 512) bb = (unsigned)v;
+        if (have) {
+            HIPCHK(hipGraphLaunch(ctx->batch_exec, ctx->stream));
+            ctx->batch_graph_launches++;
+        } else {
+            HIPCHK(enqueue());
         }
-        hipLaunchKernelGGL((k_batch_run<Alg, CP, K>), dim3((unsigned)ne), dim3(bb), 0, ctx->stream, dst);
-        HIPCHK(hipGetLastError());
         if (ctx->kernel_timing) HIPCHK(hipEventRecord(ctx->ev1, ctx->stream));
         ctx->launches += 2;
-        HIPCHK(hipMemcpyAsync(st.data(), dst, ne * sizeof(InstState<CP>), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
+        std::vector<InstState<CP>> st(hst, hst + ne);
         if (ctx->kernel_timing) {
             float ms = 0.f;
             HIPCHK(hipEventElapsedTime(&ms, ctx->ev0, ctx->ev1));
@@ -1573,6 +1654,9 @@ int run_contrib_log(ffx_ctx* ctx, std::vector<typename Engine<Alg, F64, K>::ASte
     }
     size_t want = std::max<size_t>(2 * (size_t)n_terms + 64, (size_t)1 << 16);
     for (int attempt = 0; attempt < 12; ++attempt) {
+        // the history scales with the store's capacity: start from a store sized for
+        // this log, not from whatever an earlier propagation on the context left
+        if (ctx->cap > want) free_store(ctx);
         Eng eng(ctx);
         CHK(eng.upload(hx, hz, re, im, n_terms));
         CHK(eng.reserve(want));
@@ -1773,15 +1857,13 @@ int dispatch_log_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t 
 
 template <int W>
 int dispatch_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, const ffx_options& o, int64_t slots,
-                   double* out_expect, int64_t* out_terms, int32_t* out_fallback, int prec, int K) {
-    if (prec == FFX_FP64)
-        return K == 1 ? run_pauli_batch<W, F64, 1>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback)
-                      : run_pauli_batch<W, F64, 2>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback);
-    if (prec == FFX_FP32)
-        return K == 1 ? run_pauli_batch<W, F32, 1>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback)
-                      : run_pauli_batch<W, F32, 2>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback);
-    return K == 1 ? run_pauli_batch<W, F32C, 1>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback)
-                  : run_pauli_batch<W, F32C, 2>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback);
+                   double* out_expect, int64_t* out_terms, int32_t* out_fallback, int prec, int K,
+                   const BatchConf* conf) {
+#define FFX_RB(CP_, K_) run_pauli_batch<W, CP_, K_>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback, conf)
+    if (prec == FFX_FP64) return K == 1 ? FFX_RB(F64, 1) : FFX_RB(F64, 2);
+    if (prec == FFX_FP32) return K == 1 ? FFX_RB(F32, 1) : FFX_RB(F32, 2);
+    return K == 1 ? FFX_RB(F32C, 1) : FFX_RB(F32C, 2);
+#undef FFX_RB
 }
 
 template <int W>
@@ -1815,6 +1897,6 @@ int dispatch_expect(ffx_ctx* ctx, const uint64_t* state, double* out, int kind, 
                                                float*, int64_t*, double*, int, int32_t*, int);                     \
     EXT template int dispatch_expect<W_>(ffx_ctx*, const uint64_t*, double*, int, int, int);                      \
     EXT template int dispatch_batch<W_>(ffx_ctx*, const ffx_instance*, int32_t, const ffx_options&, int64_t,       \
-                                        double*, int64_t*, int32_t*, int, int)
+                                        double*, int64_t*, int32_t*, int, int, const BatchConf*)
 
 }  // namespace ffx_host

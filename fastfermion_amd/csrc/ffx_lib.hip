@@ -70,6 +70,8 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     dev_free(ctx->batch_pool);
     dev_free(ctx->crot);
     ffx_mlp_free(ctx);
+    if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
+    if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);
@@ -289,13 +291,13 @@ int ffx_majorana_contrib_log(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64
     }
 }
 
-int ffx_pauli_propagate_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
-                              const ffx_options* opts, int64_t slots, double* out_expect, int64_t* out_terms,
-                              int32_t* out_fallback) {
-    if (!ctx) return FFX_EINVAL;
-    ctx->err.clear();
-    if (!opts || n_inst < 0 || (n_inst > 0 && (!inst || !out_expect)) || slots < 0)
-        return set_err(ctx, FFX_EINVAL, "invalid arguments");
+}  // extern "C"
+
+namespace {
+
+int check_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words, const ffx_options* opts,
+                int* K_out) {
+    if (!opts || n_inst < 0 || (n_inst > 0 && !inst)) return set_err(ctx, FFX_EINVAL, "invalid arguments");
     if (n_words < 1 || n_words > FFX_MAX_WORDS) return set_err(ctx, FFX_EINVAL, "n_words must be in 1..4");
     int K = 1;
     for (int32_t i = 0; i < n_inst; ++i) {
@@ -321,19 +323,113 @@ int ffx_pauli_propagate_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_
             for (int64_t t = 0; t < in.n_terms && K == 1; ++t)
                 if (in.c_im[t] != 0.0) K = 2;
     }
+    if (opts->precision < FFX_FP64 || opts->precision > FFX_FP32C)
+        return set_err(ctx, FFX_EINVAL, "unknown precision");
     if (hipSetDevice(ctx->device) != hipSuccess) return set_err(ctx, FFX_ENODEV, "hipSetDevice failed");
+    *K_out = K;
+    return FFX_OK;
+}
+
+int batch_impl(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words, const ffx_options& o,
+               int64_t slots, double* out_expect, int64_t* out_terms, int32_t* out_fallback, int K,
+               const BatchConf* conf) {
+    const int prec = o.precision;
+    switch (n_words) {
+        case 1: return dispatch_batch<1>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback, prec, K, conf);
+        case 2: return dispatch_batch<2>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback, prec, K, conf);
+        case 3: return dispatch_batch<3>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback, prec, K, conf);
+        default: return dispatch_batch<4>(ctx, inst, n_inst, o, slots, out_expect, out_terms, out_fallback, prec, K, conf);
+    }
+}
+
+}  // namespace
+
+extern "C" {
+
+int ffx_pauli_propagate_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
+                              const ffx_options* opts, int64_t slots, double* out_expect, int64_t* out_terms,
+                              int32_t* out_fallback) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    if (slots < 0 || (n_inst > 0 && !out_expect)) return set_err(ctx, FFX_EINVAL, "invalid arguments");
+    int K = 1;
+    CHK(check_batch(ctx, inst, n_inst, n_words, opts, &K));
     ctx->prof = ffx_profile{};
     ctx->launches = 0;
-    const int prec = opts->precision;
-    if (prec < FFX_FP64 || prec > FFX_FP32C) return set_err(ctx, FFX_EINVAL, "unknown precision");
     if (out_fallback) *out_fallback = 0;
     if (n_inst == 0) return FFX_OK;
-    switch (n_words) {
-        case 1: return dispatch_batch<1>(ctx, inst, n_inst, *opts, slots, out_expect, out_terms, out_fallback, prec, K);
-        case 2: return dispatch_batch<2>(ctx, inst, n_inst, *opts, slots, out_expect, out_terms, out_fallback, prec, K);
-        case 3: return dispatch_batch<3>(ctx, inst, n_inst, *opts, slots, out_expect, out_terms, out_fallback, prec, K);
-        default: return dispatch_batch<4>(ctx, inst, n_inst, *opts, slots, out_expect, out_terms, out_fallback, prec, K);
-    }
+    return batch_impl(ctx, inst, n_inst, n_words, *opts, slots, out_expect, out_terms, out_fallback, K, nullptr);
+}
+
+int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
+                       const ffx_options* opts, int32_t sample, ffx_batch_tuning* out) {
+    if (!ctx) return FFX_EINVAL;
+    ctx->err.clear();
+    if (n_inst <= 0 || !out) return set_err(ctx, FFX_EINVAL, "invalid arguments");
+    int K = 1;
+    CHK(check_batch(ctx, inst, n_inst, n_words, opts, &K));
+    // sample: the instances with the longest circuits (they bound a batch's time)
+    std::vector<int32_t> order((size_t)n_inst);
+    for (int32_t i = 0; i < n_inst; ++i) order[(size_t)i] = i;
+    std::stable_sort(order.begin(), order.end(),
+                     [&](int32_t a, int32_t b) { return inst[a].n_gates > inst[b].n_gates; });
+    const int32_t m = std::min<int32_t>(n_inst, sample > 0 ? sample : 128);
+    std::vector<ffx_instance> sub((size_t)m);
+    for (int32_t i = 0; i < m; ++i) sub[(size_t)i] = inst[order[(size_t)i]];
+    std::vector<double> ex(2 * (size_t)m);
+    const int blocks[3] = {128, 256, 512};
+    const int windows[2] = {3, 4};
+    const int64_t budgets[2] = {8192, 32768};
+    double best_ms = 1e300;
+    BatchConf best{};
+    int32_t tried = 0;
+    bool best_clean = false;
+    for (int bl : blocks)
+        for (int wi : windows)
+            for (int64_t sl : budgets) {
+                const BatchConf c{bl, wi, sl};
+                int32_t fb = 0;
+                // first run builds and captures; the second is timed
+                CHK(batch_impl(ctx, sub.data(), m, n_words, *opts, 0, ex.data(), nullptr, &fb, K, &c));
+                const double t0 = now_s();
+                CHK(batch_impl(ctx, sub.data(), m, n_words, *opts, 0, ex.data(), nullptr, &fb, K, &c));
+                const double ms = 1e3 * (now_s() - t0);
+                ++tried;
+                const bool clean = fb == 0;
+                if ((clean && !best_clean) || (clean == best_clean && ms < best_ms)) {
+                    best_ms = ms;
+                    best = c;
+                    best_clean = clean;
+                }
+            }
+    ffx_ctx::BatchCfg t;
+    t.W = n_words;
+    t.prec = opts->precision;
+    t.K = K;
+    t.block = best.block;
+    t.window = best.window;
+    t.slots = best.slots;
+    bool replaced = false;
+    for (auto& e : ctx->batch_tuned)
+        if (e.W == t.W && e.prec == t.prec && e.K == t.K) {
+            e = t;
+            replaced = true;
+        }
+    if (!replaced) ctx->batch_tuned.push_back(t);
+    out->block = best.block;
+    out->window = best.window;
+    out->slots = best.slots;
+    out->ms = best_ms;
+    out->candidates = tried;
+    out->sample = m;
+    ctx->prof = ffx_profile{};
+    return FFX_OK;
+}
+
+int ffx_batch_graph_replays(const ffx_ctx* ctx, int64_t* out) {
+    if (!ctx || !out) return FFX_EINVAL;
+    *out = ctx->batch_graph_launches;
+    return FFX_OK;
 }
 
 int ffx_majorana_propagate(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, int32_t n_words,

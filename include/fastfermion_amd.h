@@ -168,12 +168,38 @@ typedef struct ffx_instance {
  * run in one launch, one workgroup each, with `slots` term slots per instance
  * (0: a default).  Instances whose terms outgrow that budget, that contain
  * Clifford gates or that ask for top-k are run afterwards on the device-wide
- * engine, so every result is complete.  out_expect[2i], [2i+1] receives
+ * engine, so every result is complete.  The upload, the launch and the download
+ * of the instances' states run as one captured HIP graph, replayed when the next
+ * call has the same layout.  out_expect[2i], [2i+1] receives
  * <b_i|O_i|b_i>; out_terms[i] (optional) the live term count at the end;
  * out_fallback (optional) how many instances took the device-wide engine. */
 int ffx_pauli_propagate_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
                               const ffx_options* opts, int64_t slots, double* out_expect, int64_t* out_terms,
                               int32_t* out_fallback);
+
+/* Launch configuration of batched propagation chosen by ffx_batch_autotune. */
+typedef struct ffx_batch_tuning {
+    int32_t block;       /* threads of the workgroup that runs one instance (128, 256 or 512) */
+    int32_t window;      /* most commuting rotations fused into one window (1..4) */
+    int64_t slots;       /* term slots per instance */
+    double ms;           /* time of the sample batch with this configuration */
+    int32_t candidates;  /* configurations timed */
+    int32_t sample;      /* instances in the sample batch */
+} ffx_batch_tuning;
+
+/* Autotuner of batched propagation (no reference counterpart): times every
+ * combination of workgroup size, window length and slot budget on the `sample`
+ * instances with the longest circuits (sample <= 0: 128) with the options of
+ * `opts`, prefers configurations without device-wide fallbacks, and keeps the
+ * fastest on the context for this key width, precision and coefficient kind;
+ * later ffx_pauli_propagate_batch calls with slots = 0 use it. */
+int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
+                       const ffx_options* opts, int32_t sample, ffx_batch_tuning* out);
+
+/* Number of batched calls on the context that replayed a captured HIP graph
+ * (a batch with the same pool layout as the previous one: same instance count,
+ * schedule lengths and term counts, e.g. an angle sweep). */
+int ffx_batch_graph_replays(const ffx_ctx* ctx, int64_t* out);
 
 /* Expectation of the device-resident result of the last propagate call on a
  * product state (Pauli: qubit bits b; Majorana: mode occupations).

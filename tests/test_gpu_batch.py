@@ -143,3 +143,52 @@ def test_batch_empty_and_trivial(ctx):
     x, z, c = wl.single_pauli("Z", [0], 4)
     ex, nt, fb = ctx.pauli_propagate_batch([(np.zeros(0, _abi.PAULI_GATE_DTYPE), x, z, c, None)], 1)
     assert ex[0] == 1.0 and nt[0] == 1
+
+
+def test_graph_replays_match_direct_launches(ctx, monkeypatch):
+    """A batch whose pool layout repeats (same instances, new angles: an angle
+    sweep) replays the captured HIP graph.  Every replay, with eager work and
+    synchronisations between them, must equal the direct launch bit for bit."""
+    import math
+
+    W = 1
+    base = instances(32, 21, l_lo=3, l_span=4, d_lo=2, d_span=6)
+    kw = dict(maxdegree=8, mincoeff=1e-4, batched=False)
+
+    def sweep(scale):
+        items = []
+        for (g, x, z, c, L, d) in base:
+            g2 = g.copy()
+            rx = (g2["x"][:, 0] != 0) & (g2["z"][:, 0] == 0)
+            g2["theta"][rx] = g2["theta"][rx] * scale
+            items.append((g2, pad(x, W), pad(z, W), c, None))
+        return items
+
+    monkeypatch.setenv("FFX_BATCH_NO_GRAPH", "1")
+    direct = [ctx.pauli_propagate_batch(sweep(s), W, **kw) for s in (1.0, 0.7, 1.3)]
+    monkeypatch.delenv("FFX_BATCH_NO_GRAPH")
+    r0 = ctx.batch_graph_replays()
+    for rep in range(3):
+        for s, want in zip((1.0, 0.7, 1.3), direct):
+            got = ctx.pauli_propagate_batch(sweep(s), W, **kw)
+            assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1]) and got[2] == want[2]
+            # eager work on the same context between replays
+            g, x, z, c, L, d = base[rep]
+            ctx.pauli_propagate(g, W, pad(x, W), pad(z, W), c, download=False, want_expect=True, **kw)
+    assert ctx.batch_graph_replays() - r0 >= 8  # the first call of the run captures, the rest replay
+    assert all(math.isfinite(v.real) for v in direct[0][0])
+
+
+def test_autotune_keeps_results(ctx):
+    """The autotuned launch (workgroup size, window length, slot budget) changes
+    only speed: FP64 results stay identical to the default launch."""
+    W = 1
+    inst = instances(40, 31, l_lo=3, l_span=5, d_lo=3, d_span=8)
+    items = [(g, pad(x, W), pad(z, W), c, None) for (g, x, z, c, L, d) in inst]
+    kw = dict(maxdegree=8, mincoeff=1e-4, batched=False)
+    want = ctx.pauli_propagate_batch(items, W, **kw)
+    t = ctx.batch_autotune(items, W, sample=16, **kw)
+    assert t["block"] in (128, 256, 512) and 1 <= t["window"] <= 4 and t["slots"] >= 1024
+    assert t["candidates"] == 12 and t["sample"] == 16 and t["ms"] > 0
+    got = ctx.pauli_propagate_batch(items, W, **kw)
+    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])

@@ -264,3 +264,30 @@ def test_pruning_erases_terms(ctx):
     # every survivor of the last pruning point had |c| >= 1e-2 there; the later rotations only
     # scale or branch them, so no survivor is far below the threshold
     assert np.abs(got.c).max() > 1e-2
+
+
+def test_trained_network_ranks_terms_better_than_coefficients(ctx):
+    """What the trained network gains over truncation by |c| alone: on the logs of a
+    held-out circuit (seed not in the training set), erasing the half / four fifths
+    of the terms with the lowest predicted contribution removes far less of the
+    total contribution than erasing those with the lowest |c|."""
+    from fastfermion_amd import pruner
+
+    L = 6
+    nq, centre = L * L, (L // 2) * L + L // 2
+    x, z, c = wl.single_pauli("Z", [centre], nq)
+    row, col = wl.lattice_geometry(L)
+    po = prune_opts(row, col, centre, rot_per_layer=2 * L * (L - 1) + nq)
+    trunc = dict(maxdegree=8, mincoeff=1e-5, batched=False)
+    train = [wl.kicked_ising(L, 12, theta_lo=0.3, theta_hi=0.7, seed=s)[0] for s in range(4)]
+    F, T = pruner.pauli_logs(ctx, train, 1, x, z, c, po, log_every=10, **trunc)
+    Fh, Th = pruner.pauli_logs(ctx, [wl.kicked_ising(L, 12, theta_lo=0.3, theta_hi=0.7, seed=99)[0]], 1, x, z, c, po,
+                               log_every=10, **trunc)
+    assert len(F) > 20000 and len(Fh) > 2000
+    ctx.pruner_init(3)
+    hist = ctx.pruner_train(F, T, epochs=5, batch=2048, lr=1e-3, seed=1, under_weight=16.0)
+    assert hist[-1] < hist[0]
+    y = ctx.pruner_forward(Fh)
+    for frac in (0.5, 0.8):
+        plain, learned = pruner.removed_share(Fh, Th, y, frac)
+        assert learned < 0.2 * plain, (frac, plain, learned)
