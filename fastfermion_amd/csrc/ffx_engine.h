@@ -26,6 +26,7 @@
 
 #include "../../include/fastfermion_amd.h"
 #include "ffx_batch.h"
+#include "ffx_layer.h"
 #include "ffx_prune.h"
 
 using namespace ffx;
@@ -110,6 +111,11 @@ struct ffx_ctx {
     int64_t batch_graph_launches = 0;  // replays of a captured graph (profile)
     void* crot = nullptr;     // Clifford-rotation records of the pruned mode
     size_t crot_bytes = 0;
+    uint64_t* claim = nullptr;  // orbit claim table of the layer kernel (hslots entries)
+    size_t claim_slots = 0;
+    uint32_t layer_stamp = 0;   // stamp of the last layer launch (24 bits)
+    void* layer_buf = nullptr;  // layer records, qubit-position tables, per-rotation counters
+    size_t layer_buf_bytes = 0;
 };
 
 // Releases the pruner model of a context (defined with the pruner kernels).
@@ -414,7 +420,9 @@ struct Engine {
         n_host = ctx->ctrl_host->n;
         dead_host = ctx->ctrl_host->n_dead;
         if (ctx->ctrl_host->overflow) return set_err(ctx, FFX_EHIP, "internal error: term store overflow");
-        if (ctx->ctrl_host->index_full) return set_err(ctx, FFX_EHIP, "internal error: hash index full");
+        if (ctx->ctrl_host->index_full)
+            return set_err(ctx, FFX_EHIP,
+                           "internal error: hash index full (flags " + std::to_string(ctx->ctrl_host->index_full) + ")");
         return FFX_OK;
     }
 
@@ -677,6 +685,46 @@ struct Engine {
         return k;
     }
 
+    // Single-qubit axis of a Pauli rotation: qubit and axis (1 X, 2 Z, 3 Y); false otherwise.
+    static bool single_site(const typename Alg::Gate& g, int& site, int& axis) {
+        if constexpr (Alg::kPauli) {
+            constexpr int W = Alg::W;
+            int cnt = 0;
+            for (int w = 0; w < W; ++w) {
+                const uint64_t sup = g.p[w] | g.p[W + w];
+                cnt += __builtin_popcountll(sup);
+                if (sup) {
+                    site = 64 * w + __builtin_ctzll(sup);
+                    axis = ((g.p[w] >> (site & 63)) & 1 ? 1 : 0) | ((g.p[W + w] >> (site & 63)) & 1 ? 2 : 0);
+                }
+            }
+            return cnt == 1;
+        } else {
+            return false;
+        }
+    }
+
+    // Length of the layer starting at rotation step t: consecutive rotations on
+    // single qubits, all distinct, no compaction / top-k / pruning point inside;
+    // 0 when shorter than kLayerMin.
+    static size_t plan_layer(const std::vector<AStep>& steps, size_t t) {
+        if (steps[t].kind != 0) return 0;
+        uint64_t used[4] = {0, 0, 0, 0};
+        size_t k = 0;
+        while (t + k < steps.size() && k < (size_t)kLayerMax) {
+            const AStep& a = steps[t + k];
+            if (a.kind != 0) break;
+            // compaction points inside a layer move to its end (housekeeping only)
+            if (k > 0 && (steps[t + k - 1].topk_after || steps[t + k - 1].prune_after)) break;
+            int site = 0, axis = 0;
+            if (!single_site(a.g, site, axis) || site >= 256) break;
+            if ((used[site >> 6] >> (site & 63)) & 1) break;
+            used[site >> 6] |= 1ull << (site & 63);
+            ++k;
+        }
+        return k >= (size_t)kLayerMin ? k : 0;
+    }
+
     TopK topk_scratch() const {
         TopK tk;
         tk.hist = ctx->topk_buf;
@@ -711,7 +759,7 @@ struct Engine {
 
     int run_async(const std::vector<AStep>& steps, const std::vector<CliffGate>& cliff, double mincoeff, int maxdeg,
                   int max_xw, uint32_t topk = 0, const std::vector<CRot<Alg, CP>>* crots = nullptr,
-                  const PruneGeo* geo = nullptr, float log10_tau = 0.f) {
+                  const PruneGeo* geo = nullptr, float log10_tau = 0.f, bool layers = false) {
         if (steps.empty()) return FFX_OK;
         if (crots && !crots->empty()) {
             const size_t bytes = crots->size() * sizeof(CRot<Alg, CP>);
@@ -752,6 +800,86 @@ struct Engine {
             if (atoi(ev) > 0) win_grid = atoi(ev);
         int max_window = kWin;
         if (const char* ev = getenv("FFX_WINDOW")) max_window = std::min(kWin, std::max(1, atoi(ev)));
+        // layers of single-qubit rotations (one launch each), planned greedily from the
+        // start; overflow restarts happen at launch boundaries, so the plan stays valid
+        std::vector<uint32_t> layer_len(steps.size(), 0), layer_off(steps.size(), 0);
+        size_t layer_bytes = 0, n_layer_rot = 0;
+        if constexpr (Alg::kPauli) {
+            if (layers) {
+                for (size_t t = 0; t < steps.size();) {
+                    const size_t k = plan_layer(steps, t);
+                    if (k) {
+                        layer_len[t] = (uint32_t)k;
+                        layer_off[t] = (uint32_t)n_layer_rot;
+                        n_layer_rot += k;
+                        t += k;
+                    } else {
+                        t += steps[t].kind == 0 ? window_len(steps, t, max_window) : 1;
+                    }
+                }
+            }
+        }
+        size_t n_layers = 0;
+        for (size_t t = 0; t < steps.size(); ++t) n_layers += layer_len[t] ? 1 : 0;
+        // buffer: records (n_layer_rot), position tables (256 int16 per layer), counters (n_layer_rot int32)
+        const size_t lrec_bytes = n_layer_rot * sizeof(LayerGate<CP>);
+        const size_t lpos_off = (lrec_bytes + 255) & ~(size_t)255;
+        const size_t lsd_off = lpos_off + ((n_layers * 256 * sizeof(int16_t) + 255) & ~(size_t)255);
+        layer_bytes = lsd_off + n_layer_rot * sizeof(int32_t);
+        constexpr size_t kLM = 3 * FFX_MAX_WORDS + 8;  // per layer: mx, mz, my (4 words each), tmask, xmask
+        std::vector<uint64_t> layer_masks;
+        if (n_layers) {
+            if (layer_bytes > ctx->layer_buf_bytes) {
+                dev_free(ctx->layer_buf);
+                ctx->layer_buf_bytes = 0;
+                CHK(dev_alloc(ctx, &ctx->layer_buf, layer_bytes));
+                ctx->layer_buf_bytes = layer_bytes;
+            }
+            std::vector<char> hb(lsd_off, 0);
+            LayerGate<CP>* rec = (LayerGate<CP>*)hb.data();
+            int16_t* pos = (int16_t*)(hb.data() + lpos_off);
+            size_t li = 0;
+            for (size_t t = 0; t < steps.size(); ++t) {
+                if (!layer_len[t]) continue;
+                int16_t* pt = pos + 256 * li;
+                for (int q = 0; q < 256; ++q) pt[q] = -1;
+                uint64_t m3[kLM] = {};
+                for (uint32_t j = 0; j < layer_len[t]; ++j) {
+                    const AStep& a = steps[t + j];
+                    int site = 0, axis = 0;
+                    single_site(a.g, site, axis);
+                    LayerGate<CP>& r = rec[layer_off[t] + j];
+                    r.cs = make_fac<CP>(a.cs);
+                    r.sn = make_fac<CP>(a.sn);
+                    r.site = site;
+                    r.axis = axis;
+                    r.thresh = a.trunc_thresh;
+                    r.xw = a.trunc_xw;
+                    pt[site] = (int16_t)j;
+                    const int which = axis == 1 ? 0 : (axis == 2 ? 1 : 2);
+                    m3[which * FFX_MAX_WORDS + (site >> 6)] |= 1ull << (site & 63);
+                    if (a.trunc_thresh) m3[3 * FFX_MAX_WORDS + (j >> 6)] |= 1ull << (j & 63);
+                    if (a.trunc_xw) m3[3 * FFX_MAX_WORDS + 4 + (j >> 6)] |= 1ull << (j & 63);
+                }
+                layer_masks.insert(layer_masks.end(), m3, m3 + kLM);
+                ++li;
+            }
+            HIPCHK(hipMemcpyAsync(ctx->layer_buf, hb.data(), lsd_off, hipMemcpyHostToDevice, ctx->stream));
+            if (ctx->claim_slots != ctx->hslots || !ctx->claim) {
+                dev_free(ctx->claim);
+                ctx->claim_slots = 0;
+                CHK(dev_alloc(ctx, &ctx->claim, ctx->hslots * sizeof(uint64_t)));
+                ctx->claim_slots = ctx->hslots;
+                HIPCHK(hipMemsetAsync(ctx->claim, 0, ctx->hslots * sizeof(uint64_t), ctx->stream));
+                ctx->layer_stamp = 0;
+            }
+        }
+        std::vector<size_t> layer_index(steps.size(), 0);
+        {
+            size_t li = 0;
+            for (size_t t = 0; t < steps.size(); ++t)
+                if (layer_len[t]) layer_index[t] = li++;
+        }
         size_t s = 0;
         for (int attempt = 0; attempt < 64 && s < steps.size(); ++attempt) {
             // epochs needed from step s on
@@ -770,8 +898,20 @@ struct Engine {
             uint32_t init[2] = {n_host, dead_host};
             HIPCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
             HIPCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+            if (n_layers) {
+                HIPCHK(hipMemsetAsync((char*)ctx->layer_buf + lsd_off, 0, n_layer_rot * sizeof(int32_t), ctx->stream));
+                if (ctx->claim_slots != ctx->hslots) {
+                    dev_free(ctx->claim);
+                    ctx->claim_slots = 0;
+                    CHK(dev_alloc(ctx, &ctx->claim, ctx->hslots * sizeof(uint64_t)));
+                    ctx->claim_slots = ctx->hslots;
+                    HIPCHK(hipMemsetAsync(ctx->claim, 0, ctx->hslots * sizeof(uint64_t), ctx->stream));
+                    ctx->layer_stamp = 0;
+                }
+            }
             std::vector<int64_t> step_of(ne, -1);
             std::vector<int> is_rot(ne, 0), crot_len(ne, 0);
+            std::vector<int64_t> layer_of(ne, -1);
             std::vector<std::pair<void*, void*>> planes_at(steps.size());
             const unsigned g_cap = grid_for(ctx->cap, kTile, 4096);
             const unsigned g_idx = grid_for(std::max(ctx->cap, ctx->hslots), kBlock, 2048);
@@ -789,14 +929,53 @@ struct Engine {
             const double te0 = now_s();
             for (size_t t = s; t < steps.size();) {
                 const AStep& a = steps[t];
-                const size_t kw = a.kind == 0 ? window_len(steps, t, max_window) : 1;
+                const bool is_layer = layer_len[t] != 0;
+                const size_t kw = is_layer ? layer_len[t] : (a.kind == 0 ? window_len(steps, t, max_window) : 1);
                 planes_at[t] = {ctx->coef, ctx->coef2};
                 // every kernel_timing-th propagation launch is bracketed by events
                 const bool timed = a.kind == 0 && ctx->kernel_timing > 0 &&
                                    (ctx->launch_seq++ % (uint64_t)ctx->kernel_timing) == 0;
                 if (a.kind == 0) ctx->prof.run_launches += 1;
                 if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev], ctx->stream));
-                if (a.kind == 0 && kw > 1) {
+                if (is_layer) {
+                    if constexpr (Alg::kPauli) {
+                        constexpr int W = Alg::W;
+                        if (++ctx->layer_stamp >= (1u << 24)) {
+                            HIPCHK(hipMemsetAsync(ctx->claim, 0, ctx->hslots * sizeof(uint64_t), ctx->stream));
+                            ctx->layer_stamp = 1;
+                        }
+                        const size_t li = layer_index[t];
+                        Layer<W, CP> L{};
+                        L.gates = (const LayerGate<CP>*)ctx->layer_buf + layer_off[t];
+                        L.pos = (const int16_t*)((const char*)ctx->layer_buf + lpos_off) + 256 * li;
+                        L.sd = (int32_t*)((char*)ctx->layer_buf + lsd_off) + layer_off[t];
+                        L.claim = ctx->claim;
+                        const uint64_t* lm = layer_masks.data() + li * kLM;
+                        for (int w = 0; w < W; ++w) {
+                            L.mx[w] = lm[w];
+                            L.mz[w] = lm[FFX_MAX_WORDS + w];
+                            L.my[w] = lm[2 * FFX_MAX_WORDS + w];
+                        }
+                        for (int u = 0; u < 4; ++u) {
+                            L.tmask[u] = lm[3 * FFX_MAX_WORDS + u];
+                            L.xmask[u] = lm[3 * FFX_MAX_WORDS + 4 + u];
+                        }
+                        L.mincoeff = mincoeff;
+                        L.nr = (int)kw;
+                        L.maxdeg = maxdeg;
+                        L.max_xw = max_xw;
+                        L.stamp = ctx->layer_stamp;
+                        hipLaunchKernelGGL((k_layer_async<Alg, CP, K>), dim3(win_grid), dim3(kBlock), 0, ctx->stream,
+                                           store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L, ep, e);
+                        CHK(launch_check());
+                        if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev++ + 1], ctx->stream));
+                        std::swap(ctx->coef, ctx->coef2);
+                        step_of[e] = (int64_t)t;
+                        is_rot[e] = (int)kw;
+                        layer_of[e] = (int64_t)layer_off[t];
+                        ++e;
+                    }
+                } else if (a.kind == 0 && kw > 1) {
                     const Window<Alg, CP> w = make_window(steps, t, kw, mincoeff, maxdeg, max_xw);
                     hipLaunchKernelGGL((k_win_async<Alg, CP, K>), dim3(win_grid), dim3(kBlockWin), 0, ctx->stream,
                                        store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, w, ep, e);
@@ -875,7 +1054,10 @@ struct Engine {
                     step_of[e] = (int64_t)t;
                     ++e;
                 }
-                if (steps[t + kw - 1].compact_after) {
+                bool compact_here = steps[t + kw - 1].compact_after != 0;
+                if (is_layer)
+                    for (size_t j = 0; j < kw; ++j) compact_here = compact_here || steps[t + j].compact_after;
+                if (compact_here) {
                     const uint32_t ep_prev = e - 1;
                     hipLaunchKernelGGL((k_cmp_count<CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream, store(), ep,
                                        ep_prev, ctx->counts);
@@ -909,6 +1091,10 @@ struct Engine {
             HIPCHK(hipMemcpyAsync(hf, ep.flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(hsd.data(), ep.sd, hsd.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
                                   ctx->stream));
+            std::vector<int32_t> hlsd(n_layer_rot);
+            if (n_layer_rot)
+                HIPCHK(hipMemcpyAsync(hlsd.data(), (char*)ctx->layer_buf + lsd_off, n_layer_rot * sizeof(int32_t),
+                                      hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
             for (size_t q = 0; q < nev; ++q) {
                 float ms = 0.f;
@@ -943,12 +1129,12 @@ struct Engine {
                     st_peak = std::max<int64_t>(st_peak, live);
                     continue;
                 }
-                // window: live count entering each of its rotations from the per-rotation changes
+                // window or layer: live count entering each of its rotations from the per-rotation changes
                 int64_t l = (int64_t)hn[q] - (int64_t)hd[q];
                 for (int j = 0; j < is_rot[q]; ++j) {
                     st_rot++;
                     st_updates += l;
-                    l += hsd[(size_t)kWin * q + j];
+                    l += layer_of[q] >= 0 ? hlsd[(size_t)layer_of[q] + j] : hsd[(size_t)kWin * q + j];
                     st_outs += l;
                     st_peak = std::max<int64_t>(st_peak, l);
                 }
@@ -982,7 +1168,9 @@ struct Engine {
         if (s < steps.size()) return set_err(ctx, FFX_ECAPACITY, "term store kept overflowing");
         HIPCHK(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
-        if (ctx->ctrl_host->index_full) return set_err(ctx, FFX_EHIP, "internal error: hash index full");
+        if (ctx->ctrl_host->index_full)
+            return set_err(ctx, FFX_EHIP,
+                           "internal error: hash index full (flags " + std::to_string(ctx->ctrl_host->index_full) + ")");
         Ctrl c{};
         c.n = n_host;
         c.n_dead = dead_host;
@@ -1210,8 +1398,18 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
             prune_plan<Alg, CP, K>(sd, crots, *po);
             geo = prune_geo(*po);
         }
+        // layers need every term's degree <= kLayerMaxA: no Clifford blocks (they can raise degree),
+        // partners capped by maxdegree, observable terms within the bound
+        bool layers = cl.empty() && o.maxdegree >= 0 && o.maxdegree <= kLayerMaxA;
+        for (int64_t i = 0; i < n_terms && layers; ++i) {
+            int d = 0;
+            for (int w = 0; w < W; ++w) d += __builtin_popcountll(x[i * W + w] | z[i * W + w]);
+            layers = d <= kLayerMaxA;
+        }
+        if (const char* ev = getenv("FFX_LAYERS"))
+            if (std::strcmp(ev, "0") == 0) layers = false;
         CHK(eng.run_async(sd, cl, o.mincoeff, maxdeg, o.max_xweight, o.topk > 0 ? (uint32_t)o.topk : 0u, &crots,
-                          &geo, po ? po->log10_tau : 0.f));
+                          &geo, po ? po->log10_tau : 0.f, layers));
     } else
     for (const Step& s : steps) {
         if (s.kind == 0) {

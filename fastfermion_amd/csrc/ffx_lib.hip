@@ -69,6 +69,8 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     dev_free(ctx->topk_buf);
     dev_free(ctx->batch_pool);
     dev_free(ctx->crot);
+    dev_free(ctx->claim);
+    dev_free(ctx->layer_buf);
     ffx_mlp_free(ctx);
     if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
     if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
