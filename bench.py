@@ -42,6 +42,9 @@ def parse():
                     help="time every N-th propagation launch with HIP events (0: off)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the precision, pruned and batched side measurements")
+    ap.add_argument("--pruned", action="store_true",
+                    help="time the pruned FP32 step instead (for profiling): rotations with |cos| <= 1e-12 as "
+                         "fused Clifford maps, truncation after every rotation, mincoeff 1e-4")
     return ap.parse_args()
 
 
@@ -254,8 +257,17 @@ def main():
 
     ctx = Context(0)
     ctx.set_kernel_timing(a.timing_every)
+    step = lambda: ctx.pauli_propagate(gates, W, x, z, c, download=False, want_expect=True, **kw)  # noqa: E731
+    if a.pruned:
+        from fastfermion_amd._native import prune_opts
+
+        po = prune_opts(*_GEO["rowcol"], center, clifford_tol=1e-12, rot_per_layer=_GEO["rpl"])
+        kw = dict(maxdegree=8, mincoeff=1e-4, batched=False, precision=_abi.FP32)
+        prec = _abi.FP32
+        step = lambda: ctx.pauli_propagate_pruned(gates, W, x, z, c, po, download=False,  # noqa: E731
+                                                  want_expect=True, **kw)
     for _ in range(a.warmup):
-        ctx.pauli_propagate(gates, W, x, z, c, download=False, want_expect=True, **kw)
+        step()
     updates = outs = 0
     kernel_s = 0.0
     timed_launches = 0
@@ -264,7 +276,7 @@ def main():
     expect = None
     t0 = time.perf_counter()
     for _ in range(a.steps):
-        _, expect = ctx.pauli_propagate(gates, W, x, z, c, download=False, want_expect=True, **kw)
+        _, expect = step()
         p = ctx.profile()
         updates += p["updates"]
         outs += p["outs"]
@@ -290,7 +302,7 @@ def main():
     # HBM bytes per launch of the same kernel and workload, from the two rocprofv3
     # --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled against a plain copy kernel
     traffic = None
-    tf = os.path.join(ROOT, "profiles", "k_win_async_kicked_ising_10x10_fp64_traffic.json")
+    tf = os.path.join(ROOT, "profiles", "propagation_kicked_ising_10x10_fp64_traffic.json")
     if (L, layers, a.batched, a.precision) == (10, 20, 1, "fp64") and os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh)["bytes_per_launch"]
@@ -325,12 +337,14 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "fp32": "f32", "fp32c": "f32+f32 compensated"}[a.precision],
+        "dtype": "f32" if a.pruned else {"fp64": "f64", "fp32": "f32", "fp32c": "f32+f32 compensated"}[a.precision],
         "data": "synthetic (seeded circuit, single-Pauli observable)",
         "config": {
             "workload": f"kicked-Ising {L}x{L} lattice, {layers} layers, RZZ(-pi/2)+RX(theta_s in [0.2,0.6]), "
                         f"observable Z_{center}, maxdegree 8, mincoeff 1e-5, "
-                        f"{'batched (reference default)' if a.batched else 'per-gate'} truncation",
+                        f"{'batched (reference default)' if a.batched else 'per-gate'} truncation"
+                        + (" [pruned step: FP32, fused Clifford rotations, per-gate truncation, mincoeff 1e-4]"
+                           if a.pruned else ""),
             "qubits": nq,
             "rotations": int(len(gates)),
             "updates_per_step": updates // a.steps,
@@ -345,8 +359,8 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
-            "kernel": "k_win_async / k_rot_async (one window of up to 4 commuting rotations, or one rotation, "
-                      "per launch)",
+            "kernel": "propagation launches: k_win_async (a window of up to 4 commuting rotations) and "
+                      "k_layer_async (a layer of single-qubit rotations)",
             "kernel_ms_per_launch": 1e3 * kernel_avg,
             "alg_bytes_per_launch": bytes_per_launch,
             "launches_per_step": kernel_launches // a.steps,

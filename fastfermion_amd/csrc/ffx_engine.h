@@ -932,9 +932,11 @@ struct Engine {
                 const bool is_layer = layer_len[t] != 0;
                 const size_t kw = is_layer ? layer_len[t] : (a.kind == 0 ? window_len(steps, t, max_window) : 1);
                 planes_at[t] = {ctx->coef, ctx->coef2};
-                // every kernel_timing-th propagation launch is bracketed by events
+                // one propagation launch in kernel_timing is bracketed by events, chosen by a
+                // hash of the launch number (a fixed stride can alias with the periodic
+                // mix of window and layer launches and bias the average)
                 const bool timed = a.kind == 0 && ctx->kernel_timing > 0 &&
-                                   (ctx->launch_seq++ % (uint64_t)ctx->kernel_timing) == 0;
+                                   (hmix(0x9E3779B97F4A7C15ull * ++ctx->launch_seq) % (uint64_t)ctx->kernel_timing) == 0;
                 if (a.kind == 0) ctx->prof.run_launches += 1;
                 if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev], ctx->stream));
                 if (is_layer) {

@@ -152,8 +152,11 @@ __global__ void __launch_bounds__(kBlock) k_layer_async(Store<CP, K> st, const t
     __shared__ uint32_t scount[kLayerMaxA + 1];
     uint32_t n, dead;
     if (!step_enter(ep, e, n, dead)) return;
-    const uint32_t step = gridDim.x * blockDim.x;
-    if (blockIdx.x * blockDim.x >= n) return;  // block-uniform
+    // slots per block iteration: 64 when every slot fits the grid that way (small term
+    // sets: four waves share the orbits of one wave's slots), else one per thread
+    const uint32_t tile = (uint64_t)n <= (uint64_t)64 * gridDim.x ? 64u : blockDim.x;
+    const uint32_t step = gridDim.x * tile;
+    if (blockIdx.x * tile >= n) return;  // block-uniform
     for (int t = threadIdx.x; t < 256; t += blockDim.x) spos[t] = L.pos[t];
     for (int t = threadIdx.x; t < L.nr; t += blockDim.x) {
         sg[t] = L.gates[t];
@@ -164,27 +167,25 @@ __global__ void __launch_bounds__(kBlock) k_layer_async(Store<CP, K> st, const t
     int dd = 0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    for (uint32_t b0 = blockIdx.x * blockDim.x; b0 < n; b0 += step) {
+    for (uint32_t b0 = blockIdx.x * tile; b0 < n; b0 += step) {
         if (threadIdx.x == 0) s_cnt = 0;
         if (threadIdx.x <= kLayerMaxA) scount[threadIdx.x] = 0;
         __syncthreads();
         // ---- phase 1: own slot: copy if inactive, else claim the orbit ----
         const uint32_t i = b0 + threadIdx.x;
         bool leader = false;
-        int a_self = 0;
         uint64_t k[NP], am[W];
 #pragma unroll
         for (int q = 0; q < NP; ++q) k[q] = 0;
 #pragma unroll
         for (int w = 0; w < W; ++w) am[w] = 0;
-        if (i < n) {
+        if (threadIdx.x < tile && i < n) {
 #pragma unroll
             for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
             layer_active<W>(k, L.mx, L.mz, L.my, am);
             int a = 0;
 #pragma unroll
             for (int w = 0; w < W; ++w) a += popc(am[w]);
-            a_self = a;
             if (a == 0) {
                 // untouched by every rotation of the layer; the truncation points inside
                 // the layer still apply (to an unchanged value: once is enough)
