@@ -114,7 +114,7 @@ typedef struct ffx_profile {
     double seconds_download;
     int64_t outs;            /* sum of live terms leaving each rotation */
     int64_t compactions;     /* erased-slot compactions run */
-    int64_t run_launches;    /* launches of the propagation kernels (one rotation or one window each) */
+    int64_t run_launches;    /* launches of the propagation kernels (one rotation, window or layer each) */
     double seconds_kernel;   /* HIP-event time of the timed ones among them */
     int64_t timed_launches;  /* how many were timed (see ffx_set_kernel_timing) */
     double seconds_enqueue;  /* host time spent issuing the propagation launches */
@@ -214,11 +214,12 @@ void ffx_terms_free(ffx_terms* t);
  * src/python_bind.cpp:427-446, reports CPU phases; these are device phases). */
 int ffx_profile_get(const ffx_ctx* ctx, ffx_profile* out);
 
-/* Kernel timing (no reference counterpart): with every = N > 0, every N-th
- * propagation kernel launch is bracketed by a pair of HIP events on the
- * context's stream; ffx_profile.seconds_kernel / timed_launches report their
- * summed time and count.  Sampling keeps the event overhead out of the other
- * launches.  every = 0 switches timing off. */
+/* Kernel timing (no reference counterpart): with every = N > 0, one
+ * propagation kernel launch in N (chosen by a hash of the launch number, so the
+ * sample does not alias with periodic launch patterns) is bracketed by a pair of
+ * HIP events on the context's stream; ffx_profile.seconds_kernel /
+ * timed_launches report their summed time and count.  Sampling keeps the event
+ * overhead out of the other launches.  every = 0 switches timing off. */
 int ffx_set_kernel_timing(ffx_ctx* ctx, int every);
 
 /* Number of visible HIP devices (0 when none); never fails. */
