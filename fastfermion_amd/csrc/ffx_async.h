@@ -23,14 +23,27 @@ namespace ffx {
 
 constexpr int kBlockRot = 1024;
 
+// Counters that every block of a launch adds to, and that only the host reads
+// back, are striped over kStripes copies one cache line or more apart (copy
+// blockIdx % kStripes): atomics on one cache line serialize (about 11 ns each),
+// so 1536 blocks adding five counters of one line cost ~90 us, striped ~3 us.
+constexpr int kStripes = 32;
+
 struct Epochs {
     uint32_t* n;     // slots in use at the start of the epoch's kernel
     uint32_t* dead;  // erased slots at the start
     uint32_t* app;   // appends of the epoch
     int32_t* dd;     // erased-count change of the epoch
     uint32_t* flags; // [0] overflow raised, [1] epoch that raised it
-    int32_t* sd;     // windows: live-count change at each rotation of the epoch's window
+    int32_t* sd;     // windows: live-count change at each rotation of the epoch's window,
+                     // kStripes copies sd_stride words apart (summed by the host)
+    uint32_t sd_stride;
 };
+
+// This block's copy of the striped per-rotation counters.
+__device__ __forceinline__ int32_t* sd_stripe(const Epochs& ep, uint32_t copy) {
+    return ep.sd + (size_t)(copy % kStripes) * ep.sd_stride;
+}
 
 __device__ __forceinline__ bool run_stopped(const Epochs& ep) { return atomic_load_u32(&ep.flags[0]) != 0; }
 
@@ -165,8 +178,7 @@ __global__ void __launch_bounds__(kBlock) k_cmp_scatter(Store<CP, K> st, uint64_
             uint32_t tot;
             const uint32_t pos = run + block_exclusive_scan(live ? 1u : 0u, &tot);
             if (live) {
-#pragma unroll
-                for (int q = 0; q < NP; ++q) key2[q * cap + pos] = st.key[q * cap + i];
+                key_copy<NP>(key2, pos, st.key, i);
 #pragma unroll
                 for (int kk = 0; kk < K; ++kk)
                     CP::store(coef2 + (size_t)kk * CP::NS * cap, cap, pos, st.ld(st.coef, kk, i));
@@ -189,8 +201,7 @@ __global__ void __launch_bounds__(kBlock) k_cmp_copyback(Store<CP, K> st, const 
     const size_t cap = st.cap;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += stride) {
-#pragma unroll
-        for (int q = 0; q < NP; ++q) st.key[q * cap + i] = key2[q * cap + i];
+        key_copy<NP>(st.key, i, key2, i);
 #pragma unroll
         for (int kk = 0; kk < K * CP::NS; ++kk) st.coef[kk * cap + i] = coef2[kk * cap + i];
     }
@@ -210,8 +221,7 @@ __global__ void __launch_bounds__(kBlock) k_index_build_async(const uint64_t* ke
     const uint32_t n = ep.n[e];
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t k[NP];
-#pragma unroll
-        for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+        key_load<NP>(key, i, k);
         index_insert(index, hmask, key_hash<NP>(k), i, ierr);
     }
 }
@@ -231,11 +241,13 @@ __global__ void __launch_bounds__(kBlock) k_cliff_async(Store<CP, K> st, const C
     const size_t cap = st.cap;
     const size_t stride = (size_t)gridDim.x * blockDim.x;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint64_t kk2[2 * W];
+        key_load<2 * W>(st.key, i, kk2);
         uint64_t x[W], z[W];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            x[w] = st.key[w * cap + i];
-            z[w] = st.key[(W + w) * cap + i];
+            x[w] = kk2[w];
+            z[w] = kk2[W + w];
         }
         int sgn = 1;
         for (int t = ng - 1; t >= 0; --t) {
@@ -280,9 +292,10 @@ __global__ void __launch_bounds__(kBlock) k_cliff_async(Store<CP, K> st, const C
         }
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            st.key[w * cap + i] = x[w];
-            st.key[(W + w) * cap + i] = z[w];
+            kk2[w] = x[w];
+            kk2[W + w] = z[w];
         }
+        key_store<2 * W>(st.key, i, kk2);
         if (sgn < 0 && !CP::dead(st.ld(st.coef, 0, i))) {
 #pragma unroll
             for (int kk = 0; kk < K; ++kk) st.st(st.coef, kk, i, CP::neg(st.ld(st.coef, kk, i)));
@@ -394,8 +407,7 @@ __global__ void __launch_bounds__(kBlock) k_topk_apply(Store<CP, K> st, Epochs e
         }
         if (!kill && xw) {
             uint64_t k[NP];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
+            key_load<NP>(st.key, i, k);
             kill = Alg::xweight(k) > max_xw;
         }
         if (kill) {

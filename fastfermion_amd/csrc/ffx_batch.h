@@ -68,8 +68,7 @@ __global__ void __launch_bounds__(kBlockBatch) k_batch_run(InstState<CP>* insts)
         __syncthreads();
         for (uint32_t i = tid; i < n; i += bd) {
             uint64_t k[NP];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+            key_load<NP>(key, i, k);
             index_insert(index, hmask, key_hash<NP>(k), i, &I->ctrl.index_full);
         }
         __syncthreads();
@@ -94,7 +93,9 @@ __global__ void __launch_bounds__(kBlockBatch) k_batch_run(InstState<CP>* insts)
         const AppendSink sink{&s_app, &s_flag, nullptr, 0};
         int dd = 0;
         int sdl[kWin] = {0, 0, 0, 0};
-        for (uint32_t b0 = 0; b0 < n; b0 += bd) win_tile<Alg, CP, K>(st, cin, cout, w, n, b0, sink, &sbase, dd, sdl);
+        for (uint32_t b0 = 0; b0 < n; b0 += bd)
+            win_tile<Alg, CP, K>(st, cin, cout, w, n, b0, sink, &sbase, dd, sdl, nullptr, false,
+                                 SlotLoad<NP, CP, K>{});
         int r[1 + kWin] = {dd, sdl[0], sdl[1], sdl[2], sdl[3]};
         block_sum_ints<1 + kWin>(r);
         if (tid == 0) {
@@ -130,8 +131,7 @@ __global__ void __launch_bounds__(kBlockBatch) k_batch_run(InstState<CP>* insts)
                 uint32_t tot;
                 const uint32_t pos = base + block_exclusive_scan(live ? 1u : 0u, &tot);
                 if (live) {
-#pragma unroll
-                    for (int q = 0; q < NP; ++q) key2[q * cap + pos] = key[q * cap + i];
+                    key_copy<NP>(key2, pos, key, i);
 #pragma unroll
                     for (int kk = 0; kk < K; ++kk)
                         CP::store(cout + (size_t)kk * NS * cap, cap, pos, CP::load(cin + (size_t)kk * NS * cap, cap, i));
@@ -157,8 +157,7 @@ __global__ void __launch_bounds__(kBlockBatch) k_batch_run(InstState<CP>* insts)
         const typename CP::val_t c0 = CP::load(cin, cap, i);
         if (CP::dead(c0)) continue;
         uint64_t k[NP];
-#pragma unroll
-        for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+        key_load<NP>(key, i, k);
         const int f = state_factor<Alg>(k, I->state);
         if (f) {
             acc[0] += f * CP::to_double(c0);
@@ -191,7 +190,7 @@ __global__ void __launch_bounds__(kBlockBatch) k_batch_run(InstState<CP>* insts)
 }
 
 // Input terms of every instance (term-major x/z words and coefficients,
-// instance b owns terms [off[b], off[b+1])) into the instances' SoA stores.
+// instance b owns terms [off[b], off[b+1])) into the instances' stores.
 template <int W, class CP, int K>
 __global__ void __launch_bounds__(kBlock) k_batch_unpack(InstState<CP>* insts, const uint64_t* x, const uint64_t* z,
                                                          const double* re, const double* im, const int64_t* off) {
@@ -200,11 +199,13 @@ __global__ void __launch_bounds__(kBlock) k_batch_unpack(InstState<CP>* insts, c
     const int64_t a = off[blockIdx.x], b = off[blockIdx.x + 1];
     for (int64_t t = a + threadIdx.x; t < b; t += blockDim.x) {
         const size_t j = (size_t)(t - a);
+        uint64_t k[2 * W];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            I->key[w * cap + j] = x[t * W + w];
-            I->key[(W + w) * cap + j] = z[t * W + w];
+            k[w] = x[t * W + w];
+            k[W + w] = z[t * W + w];
         }
+        key_store<2 * W>(I->key, j, k);
         CP::store(I->coef, cap, j, CP::from_double(re[t]));
         if constexpr (K == 2) CP::store(I->coef + (size_t)CP::NS * cap, cap, j, CP::from_double(im ? im[t] : 0.0));
     }

@@ -1,9 +1,12 @@
 // ffx_device.h -- device-side building blocks of the propagation engine.
 //
 // Term store (per observable, capacity `cap` slots, structure of arrays):
-//   key   : NP planes of uint64, plane p of slot i at key[p*cap + i]
-//           Pauli<W>: planes 0..W-1 = xory words, W..2W-1 = yorz words
-//           Majorana<W>: planes 0..W-1 = occupation words
+//   key   : one record of NP uint64 words per slot, word p of slot i at
+//           key[i*NP + p] (a lookup that verifies a key reads one 8*NP-byte
+//           record, not NP scattered words; a wave streaming keys still reads
+//           one contiguous 64*8*NP-byte run)
+//           Pauli<W>: words 0..W-1 = xory, W..2W-1 = yorz
+//           Majorana<W>: words 0..W-1 = occupation
 //   coef  : K coefficient planes (K = 1 real, 2 complex); each plane holds
 //           CP::NS storage slots per value (1 for FP64/FP32, 2 for FP32C)
 //   index : open-addressing hash table, 64-bit entries (tag << 32 | slot)
@@ -76,6 +79,42 @@ __device__ __forceinline__ uint64_t key_hash(const uint64_t (&k)[NP]) {
 }
 
 __device__ __forceinline__ int popc(uint64_t v) { return __popcll(v); }
+
+// Key record of slot i.  Records of an even word count are 16-byte aligned
+// (every key array starts 16-byte aligned) and move as 16-byte accesses.
+template <int NP>
+__device__ __forceinline__ void key_load(const uint64_t* __restrict__ key, size_t i, uint64_t (&k)[NP]) {
+    const uint64_t* p = key + i * NP;
+    if constexpr ((NP & 1) == 0) {
+#pragma unroll
+        for (int q = 0; q < NP; q += 2) {
+            const ulonglong2 v = *reinterpret_cast<const ulonglong2*>(p + q);
+            k[q] = v.x;
+            k[q + 1] = v.y;
+        }
+    } else {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) k[q] = p[q];
+    }
+}
+template <int NP>
+__device__ __forceinline__ void key_store(uint64_t* __restrict__ key, size_t i, const uint64_t (&k)[NP]) {
+    uint64_t* p = key + i * NP;
+    if constexpr ((NP & 1) == 0) {
+#pragma unroll
+        for (int q = 0; q < NP; q += 2) *reinterpret_cast<ulonglong2*>(p + q) = make_ulonglong2(k[q], k[q + 1]);
+    } else {
+#pragma unroll
+        for (int q = 0; q < NP; ++q) p[q] = k[q];
+    }
+}
+template <int NP>
+__device__ __forceinline__ void key_copy(uint64_t* __restrict__ dst, size_t j, const uint64_t* __restrict__ src,
+                                         size_t i) {
+    uint64_t k[NP];
+    key_load<NP>(src, i, k);
+    key_store<NP>(dst, j, k);
+}
 
 // ---------------------------------------------------------------------------
 // Coefficient policies
@@ -330,13 +369,11 @@ __device__ __forceinline__ uint32_t probe(const uint64_t* __restrict__ key, size
         if (e == kEmpty) return kNone;
         const uint32_t idx = (uint32_t)e;
         if ((uint32_t)(e >> 32) == tag && idx < n) {
+            uint64_t c[NP];
+            key_load<NP>(key, idx, c);
             bool eq = true;
 #pragma unroll
-            for (int q = 0; q < NP; ++q)
-                if (key[q * cap + idx] != r[q]) {
-                    eq = false;
-                    break;
-                }
+            for (int q = 0; q < NP; ++q) eq = eq && c[q] == r[q];
             if (eq) return idx;
         }
         p = (p + 1) & hmask;

@@ -383,9 +383,8 @@ struct Engine {
             return rc;
         }
         if (n_host > 0 && ctx->key && ctx->key_planes == NP && ctx->coef_planes == coef_planes()) {
-            for (int q = 0; q < NP; ++q)
-                HIPCHK(hipMemcpyAsync(nkey + q * ncap, ctx->key + q * ctx->cap, n_host * sizeof(uint64_t),
-                                      hipMemcpyDeviceToDevice, ctx->stream));
+            HIPCHK(hipMemcpyAsync(nkey, ctx->key, n_host * NP * sizeof(uint64_t), hipMemcpyDeviceToDevice,
+                                  ctx->stream));
             for (size_t q = 0; q < coef_planes(); ++q)
                 HIPCHK(hipMemcpyAsync((store_t*)ncoef + q * ncap, (store_t*)ctx->coef + q * ctx->cap,
                                       n_host * sizeof(store_t), hipMemcpyDeviceToDevice, ctx->stream));
@@ -613,14 +612,15 @@ struct Engine {
         if (ne <= ctx->epochs_cap) return FFX_OK;
         dev_free(ctx->epochs);
         ctx->epochs_cap = 0;
-        const size_t c = pow2ceil(ne);
+        const size_t c = std::max<size_t>(pow2ceil(ne), 64);  // stripe copies >= 1 KiB apart
         CHK(dev_alloc(ctx, &ctx->epochs, epoch_words(c) * sizeof(uint32_t)));
         ctx->epochs_cap = c;
         return FFX_OK;
     }
 
-    // n, dead, app, dd (one word per epoch each), flags (64 words), sd (kWin per epoch)
-    static size_t epoch_words(size_t c) { return (4 + kWin) * c + 64; }
+    // n, dead, app, dd (one word per epoch each), flags (64 words), sd (kWin per epoch,
+    // kStripes copies)
+    static size_t epoch_words(size_t c) { return (4 + (size_t)kStripes * kWin) * c + 64; }
 
     Epochs epochs() const {
         Epochs ep;
@@ -631,6 +631,7 @@ struct Engine {
         ep.dd = (int32_t*)(ctx->epochs + 3 * c);
         ep.flags = ctx->epochs + 4 * c;
         ep.sd = (int32_t*)(ctx->epochs + 4 * c + 64);
+        ep.sd_stride = (uint32_t)(kWin * c);
         return ep;
     }
 
@@ -825,7 +826,8 @@ struct Engine {
         const size_t lrec_bytes = n_layer_rot * sizeof(LayerGate<CP>);
         const size_t lpos_off = (lrec_bytes + 255) & ~(size_t)255;
         const size_t lsd_off = lpos_off + ((n_layers * 256 * sizeof(int16_t) + 255) & ~(size_t)255);
-        layer_bytes = lsd_off + n_layer_rot * sizeof(int32_t);
+        const size_t lsd_stride = (n_layer_rot + 63) & ~(size_t)63;  // striped counter copies
+        layer_bytes = lsd_off + kStripes * lsd_stride * sizeof(int32_t);
         constexpr size_t kLM = 3 * FFX_MAX_WORDS + 8;  // per layer: mx, mz, my (4 words each), tmask, xmask
         std::vector<uint64_t> layer_masks;
         if (n_layers) {
@@ -899,7 +901,8 @@ struct Engine {
             HIPCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
             HIPCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
             if (n_layers) {
-                HIPCHK(hipMemsetAsync((char*)ctx->layer_buf + lsd_off, 0, n_layer_rot * sizeof(int32_t), ctx->stream));
+                HIPCHK(hipMemsetAsync((char*)ctx->layer_buf + lsd_off, 0, kStripes * lsd_stride * sizeof(int32_t),
+                                      ctx->stream));
                 if (ctx->claim_slots != ctx->hslots) {
                     dev_free(ctx->claim);
                     ctx->claim_slots = 0;
@@ -951,6 +954,7 @@ struct Engine {
                         L.gates = (const LayerGate<CP>*)ctx->layer_buf + layer_off[t];
                         L.pos = (const int16_t*)((const char*)ctx->layer_buf + lpos_off) + 256 * li;
                         L.sd = (int32_t*)((char*)ctx->layer_buf + lsd_off) + layer_off[t];
+                        L.sd_stride = (uint32_t)lsd_stride;
                         L.claim = ctx->claim;
                         const uint64_t* lm = layer_masks.data() + li * kLM;
                         for (int w = 0; w < W; ++w) {
@@ -1084,20 +1088,29 @@ struct Engine {
             ctx->prof.seconds_enqueue += now_s() - te0;
             // one sync per run: epochs, flags
             std::vector<uint32_t> hn(e), hd(e), ha(e);
-            std::vector<int32_t> hdd(e), hsd((size_t)kWin * e);
+            std::vector<int32_t> hdd(e), hsd_all((size_t)kStripes * kWin * e);
             uint32_t hf[2] = {0, 0};
             HIPCHK(hipMemcpyAsync(hn.data(), ep.n, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(hd.data(), ep.dead, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(ha.data(), ep.app, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(hdd.data(), ep.dd, e * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(hf, ep.flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpyAsync(hsd.data(), ep.sd, hsd.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
-                                  ctx->stream));
-            std::vector<int32_t> hlsd(n_layer_rot);
+            HIPCHK(hipMemcpy2DAsync(hsd_all.data(), (size_t)kWin * e * sizeof(int32_t), ep.sd,
+                                    (size_t)ep.sd_stride * sizeof(int32_t), (size_t)kWin * e * sizeof(int32_t),
+                                    kStripes, hipMemcpyDeviceToHost, ctx->stream));
+            std::vector<int32_t> hlsd_all((size_t)kStripes * n_layer_rot);
             if (n_layer_rot)
-                HIPCHK(hipMemcpyAsync(hlsd.data(), (char*)ctx->layer_buf + lsd_off, n_layer_rot * sizeof(int32_t),
-                                      hipMemcpyDeviceToHost, ctx->stream));
+                HIPCHK(hipMemcpy2DAsync(hlsd_all.data(), n_layer_rot * sizeof(int32_t),
+                                        (char*)ctx->layer_buf + lsd_off, lsd_stride * sizeof(int32_t),
+                                        n_layer_rot * sizeof(int32_t), kStripes, hipMemcpyDeviceToHost,
+                                        ctx->stream));
             HIPCHK(hipStreamSynchronize(ctx->stream));
+            // sums of the striped counter copies
+            std::vector<int32_t> hsd((size_t)kWin * e, 0), hlsd(n_layer_rot, 0);
+            for (int c = 0; c < kStripes; ++c) {
+                for (size_t j = 0; j < hsd.size(); ++j) hsd[j] += hsd_all[(size_t)c * kWin * e + j];
+                for (size_t j = 0; j < n_layer_rot; ++j) hlsd[j] += hlsd_all[(size_t)c * n_layer_rot + j];
+            }
             for (size_t q = 0; q < nev; ++q) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, ctx->kev[2 * q], ctx->kev[2 * q + 1]));
@@ -1111,7 +1124,11 @@ struct Engine {
                     // start, the rotations each one survives; a term erased inside the run also
                     // entered the rotation that erased it
                     uint64_t surv = 0;
-                    std::memcpy(&surv, &hsd[(size_t)kWin * q], sizeof(surv));
+                    for (int c = 0; c < kStripes; ++c) {
+                        uint64_t part = 0;
+                        std::memcpy(&part, &hsd_all[(size_t)c * kWin * e + (size_t)kWin * q], sizeof(part));
+                        surv += part;
+                    }
                     const int64_t lout = (int64_t)(hn[q] + ha[q]) - (int64_t)((int32_t)hd[q] + hdd[q]);
                     st_rot += crot_len[q];
                     st_updates += (int64_t)surv + hdd[q];

@@ -175,8 +175,7 @@ __device__ __forceinline__ void rot_slot(const Store<CP, K>& st, const uint64_t*
     using val_t = typename CP::val_t;
     const size_t cap = st.cap;
     uint64_t k[NP];
-#pragma unroll
-    for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+    key_load<NP>(key, i, k);
     val_t c[K];
 #pragma unroll
     for (int kk = 0; kk < K; ++kk) c[kk] = CP::load(cin + (size_t)kk * NS * cap, cap, i);
@@ -238,8 +237,7 @@ __device__ __forceinline__ void put_slot(const Store<CP, K>& st, uint64_t* key, 
                                          uint64_t slot, const uint64_t (&r)[NP],
                                          const typename CP::val_t (&vapp)[K]) {
     const size_t cap = st.cap;
-#pragma unroll
-    for (int q = 0; q < NP; ++q) key[q * cap + slot] = r[q];
+    key_store<NP>(key, slot, r);
 #pragma unroll
     for (int kk = 0; kk < K; ++kk) CP::store(cout + (size_t)kk * CP::NS * cap, cap, slot, vapp[kk]);
     index_insert(st.index, st.hmask, key_hash<NP>(r), (uint32_t)slot, &st.ctrl->index_full);
@@ -251,8 +249,7 @@ __device__ __forceinline__ void put_slot_h(const Store<CP, K>& st, uint64_t* key
                                            uint64_t slot, const uint64_t (&r)[NP],
                                            const typename CP::val_t (&vapp)[K], uint64_t h) {
     const size_t cap = st.cap;
-#pragma unroll
-    for (int q = 0; q < NP; ++q) key[q * cap + slot] = r[q];
+    key_store<NP>(key, slot, r);
 #pragma unroll
     for (int kk = 0; kk < K; ++kk) CP::store(cout + (size_t)kk * CP::NS * cap, cap, slot, vapp[kk]);
     index_insert(st.index, st.hmask, h, (uint32_t)slot, &st.ctrl->index_full);
@@ -321,11 +318,13 @@ __global__ void __launch_bounds__(kBlock) k_clifford(Store<CP, K> st, const Clif
     const uint32_t n = st.ctrl->n;
     const size_t cap = st.cap;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
+        uint64_t kk2[2 * W];
+        key_load<2 * W>(st.key, i, kk2);
         uint64_t x[W], z[W];
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            x[w] = st.key[w * cap + i];
-            z[w] = st.key[(W + w) * cap + i];
+            x[w] = kk2[w];
+            z[w] = kk2[W + w];
         }
         int sgn = 1;
         for (int t = ng - 1; t >= 0; --t) {
@@ -372,9 +371,10 @@ __global__ void __launch_bounds__(kBlock) k_clifford(Store<CP, K> st, const Clif
         }
 #pragma unroll
         for (int w = 0; w < W; ++w) {
-            st.key[w * cap + i] = x[w];
-            st.key[(W + w) * cap + i] = z[w];
+            kk2[w] = x[w];
+            kk2[W + w] = z[w];
         }
+        key_store<2 * W>(st.key, i, kk2);
         if (sgn < 0) {
             typename CP::val_t c0 = st.ld(st.coef, 0, i);
             if (!CP::dead(c0)) {
@@ -398,8 +398,7 @@ __global__ void __launch_bounds__(kBlock) k_index_build(const uint64_t* key, siz
                                                         uint64_t hmask, Ctrl* ctrl, uint32_t n) {
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t k[NP];
-#pragma unroll
-        for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+        key_load<NP>(key, i, k);
         index_insert(index, hmask, key_hash<NP>(k), i, &ctrl->index_full);
     }
 }
@@ -411,8 +410,7 @@ __global__ void __launch_bounds__(kBlock) k_index_check(const uint64_t* key, siz
     const uint32_t n = ctrl->n;
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t k[NP];
-#pragma unroll
-        for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+        key_load<NP>(key, i, k);
         if (probe<NP>(key, cap, index, hmask, k, n, &ctrl->index_full) != i) atomicOr(&ctrl->dup, 1u);
     }
 }
@@ -463,8 +461,7 @@ __global__ void __launch_bounds__(kBlock) k_compact(Store<CP, K> st, const uint3
         uint32_t tot;
         const uint32_t pos = run + block_exclusive_scan(live ? 1u : 0u, &tot);
         if (live) {
-#pragma unroll
-            for (int q = 0; q < NP; ++q) key_dst[q * cap + pos] = st.key[q * cap + i];
+            key_copy<NP>(key_dst, pos, st.key, i);
 #pragma unroll
             for (int kk = 0; kk < K; ++kk)
                 CP::store(coef_dst + (size_t)kk * CP::NS * cap, cap, pos, st.ld(st.coef, kk, i));
@@ -474,7 +471,7 @@ __global__ void __launch_bounds__(kBlock) k_compact(Store<CP, K> st, const uint3
 }
 
 // ---------------------------------------------------------------------------
-// Upload / download conversion (term-major host layout <-> planar device layout)
+// Upload / download conversion (host x/z word arrays <-> device key records)
 // Majorana coefficients are stored normalized: d = c / i^{r}, r = 1 iff the
 // string degree is 2 or 3 mod 4, which keeps Hermitian observables real.
 // ---------------------------------------------------------------------------
@@ -490,8 +487,7 @@ __global__ void __launch_bounds__(kBlock) k_unpack(Store<CP, K> st, const uint64
             k[w] = hx[(size_t)i * W + w];
             if constexpr (Alg::kPauli) k[W + w] = hz[(size_t)i * W + w];
         }
-#pragma unroll
-        for (int q = 0; q < Alg::NP; ++q) st.key[q * cap + i] = k[q];
+        key_store<Alg::NP>(st.key, i, k);
         double a = re[i], b = im ? im[i] : 0.0;
         if constexpr (!Alg::kPauli) {
             if ((Alg::degree(k) >> 1) & 1) {  // d = c / i = (b, -a)
@@ -525,8 +521,7 @@ __global__ void __launch_bounds__(kBlock) k_pack(Store<CP, K> st, const uint32_t
         const uint32_t pos = run + block_exclusive_scan(live ? 1u : 0u, &tot);
         if (live) {
             uint64_t k[Alg::NP];
-#pragma unroll
-            for (int q = 0; q < Alg::NP; ++q) k[q] = st.key[q * cap + i];
+            key_load<Alg::NP>(st.key, i, k);
 #pragma unroll
             for (int w = 0; w < W; ++w) {
                 ox[(size_t)pos * W + w] = k[w];
@@ -601,8 +596,7 @@ __global__ void __launch_bounds__(kBlock) k_expect_partial(Store<CP, K> st, Stat
         const typename CP::val_t c0 = st.ld(st.coef, 0, i);
         if (CP::dead(c0)) continue;
         uint64_t k[Alg::NP];
-#pragma unroll
-        for (int q = 0; q < Alg::NP; ++q) k[q] = st.key[q * cap + i];
+        key_load<Alg::NP>(st.key, i, k);
         const int f = state_factor<Alg>(k, sw.s);
         if (f) {
             acc[0] += f * CP::to_double(c0);

@@ -52,7 +52,9 @@ template <int W, class CP>
 struct Layer {
     const LayerGate<CP>* gates;  // nr records in layer order
     const int16_t* pos;          // qubit -> position in the layer, -1: none (256 entries)
-    int32_t* sd;                 // live-count change at each rotation of the layer (nr entries)
+    int32_t* sd;                 // live-count change at each rotation of the layer (nr entries),
+                                 // kStripes copies sd_stride words apart (summed by the host)
+    uint32_t sd_stride;
     uint64_t* claim;             // claim table (hmask + 1 entries)
     uint64_t mx[W], mz[W], my[W];  // qubits whose axis is X / Z / Y
     uint64_t tmask[4], xmask[4];   // layer positions followed by the threshold / by the X/Y cap
@@ -180,8 +182,7 @@ __global__ void __launch_bounds__(kBlock) k_layer_async(Store<CP, K> st, const t
 #pragma unroll
         for (int w = 0; w < W; ++w) am[w] = 0;
         if (threadIdx.x < tile && i < n) {
-#pragma unroll
-            for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
+            key_load<NP>(st.key, i, k);
             layer_active<W>(k, L.mx, L.mz, L.my, am);
             int a = 0;
 #pragma unroll
@@ -249,8 +250,7 @@ __global__ void __launch_bounds__(kBlock) k_layer_async(Store<CP, K> st, const t
                     if (((e0 >> 32) & 0xFFull) == tag8) {
                         const uint32_t j = (uint32_t)e0;
                         uint64_t kj[NP], amj[W], ckj[NP];
-#pragma unroll
-                        for (int q = 0; q < NP; ++q) kj[q] = st.key[q * cap + j];
+                        key_load<NP>(st.key, j, kj);
                         layer_active<W>(kj, L.mx, L.mz, L.my, amj);
                         layer_canon<W>(kj, amj, L.mx, L.mz, L.my, ckj);
                         bool eq = true;
@@ -659,8 +659,9 @@ __global__ void __launch_bounds__(kBlock) k_layer_async(Store<CP, K> st, const t
     }
     dd = block_sum_int(dd);
     if (threadIdx.x == 0 && dd) atomicAdd(&ep.dd[e], dd);
+    int32_t* sd = L.sd + (size_t)(blockIdx.x % kStripes) * L.sd_stride;
     for (int t = threadIdx.x; t < L.nr; t += blockDim.x)
-        if (ssd[t]) atomicAdd(&L.sd[t], ssd[t]);
+        if (ssd[t]) atomicAdd(&sd[t], ssd[t]);
 }
 
 }  // namespace ffx

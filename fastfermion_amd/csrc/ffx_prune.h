@@ -67,8 +67,7 @@ __global__ void __launch_bounds__(kBlock) k_crot_async(Store<CP, K> st, const CR
     unsigned long long surv = 0;  // sum over live terms of the rotations they survive
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         uint64_t k[NP];
-#pragma unroll
-        for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
+        key_load<NP>(st.key, i, k);
         val_t c[K];
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) c[kk] = st.ld(st.coef, kk, i);
@@ -98,8 +97,7 @@ __global__ void __launch_bounds__(kBlock) k_crot_async(Store<CP, K> st, const CR
                 if (!kill) ++surv;
             }
         }
-#pragma unroll
-        for (int q = 0; q < NP; ++q) st.key[q * cap + i] = k[q];
+        key_store<NP>(st.key, i, k);
         if (alive) {
             if (kill) {
                 st.st(st.coef, 0, i, CP::make_dead());
@@ -117,7 +115,10 @@ __global__ void __launch_bounds__(kBlock) k_crot_async(Store<CP, K> st, const CR
     dd = block_sum_int(dd);
     // rotation-survival count of the block, 64-bit, into the epoch's sd words (read by the host)
     for (int o = 32; o > 0; o >>= 1) surv += __shfl_down(surv, o, 64);
-    if (__lane_id() == 0 && surv) atomicAdd((unsigned long long*)(ep.sd + (size_t)kWin * e), surv);
+    if (__lane_id() == 0 && surv)
+        atomicAdd((unsigned long long*)(sd_stripe(ep, blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) +
+                                        (size_t)kWin * e),
+                  surv);
     if (threadIdx.x == 0 && dd) atomicAdd(&ep.dd[e], dd);
 }
 
@@ -388,8 +389,7 @@ __global__ void __launch_bounds__(64 * kPruneWaves) k_prune_async(Store<CP, 1> s
             live = !CP::dead(c);
             if (live) {
                 uint64_t k[NP];
-#pragma unroll
-                for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
+                key_load<NP>(st.key, i, k);
                 const double a = fabs(CP::to_double(c));
                 logc = a > 0.0 ? (float)log10(a) : -300.f;
                 term_features<Alg>(k, logc, sgeo, fc, F + l * kS1);
@@ -445,8 +445,7 @@ __global__ void __launch_bounds__(kBlock) k_adj_init(const uint64_t* __restrict_
         double val = 0.0;
         if (bit_of(bits, i)) {
             uint64_t k[Alg::NP];
-#pragma unroll
-            for (int q = 0; q < Alg::NP; ++q) k[q] = key[q * cap + i];
+            key_load<Alg::NP>(key, i, k);
             val = (double)state_factor<Alg>(k, sw.s);
         }
         v[i] = val;
@@ -470,8 +469,7 @@ __global__ void __launch_bounds__(kBlock) k_adjoint(const uint64_t* __restrict__
         double v = 0.0;
         if (bit_of(bits_in, i)) {
             uint64_t k[NP];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+            key_load<NP>(key, i, k);
             v = v_out[i];
             if (Alg::anticommutes(k, g)) {
                 v = v * cs;
@@ -512,8 +510,7 @@ __global__ void __launch_bounds__(kBlock) k_log_extract(const uint64_t* __restri
         const uint32_t i = b0 + threadIdx.x;
         if (i < n && bit_of(bits, i)) {
             uint64_t k[NP];
-#pragma unroll
-            for (int q = 0; q < NP; ++q) k[q] = key[q * cap + i];
+            key_load<NP>(key, i, k);
             const double cr = coef[i];
             const double ci = K == 2 ? coef[cap + i] : 0.0;
             const double vi = v[i];

@@ -110,10 +110,45 @@ __device__ __forceinline__ void member_facts(const uint64_t (&kt)[Alg::NP], int 
     f.qm |= (uint32_t)Alg::quad(kt) << (2 * T);
 }
 
+// Continues the lookup of kt (index hash h) past its home entry, which was
+// examined and does not hold it, and (kValue) loads the candidate's value
+// together with its key words.  Returns the slot among [0, n), or kNone.
+template <class Alg, class CP, int K, bool kValue>
+__device__ __forceinline__ uint32_t probe_next(const Store<CP, K>& st, const typename CP::store_t* cin,
+                                               const uint64_t (&kt)[Alg::NP], uint64_t h, uint32_t n,
+                                               typename CP::val_t (&vt)[K]) {
+    constexpr int NP = Alg::NP;
+    const size_t cap = st.cap;
+    const uint32_t tag = (uint32_t)(h >> 32);
+    uint64_t p = h & st.hmask;
+    for (uint64_t step = 1;; ++step) {
+        if (step > st.hmask) {
+            atomicOr(&st.ctrl->index_full, 1u);
+            return kNone;
+        }
+        p = (p + 1) & st.hmask;
+        const uint64_t e = st.index[p];
+        if (e == kEmpty) return kNone;
+        const uint32_t idx = (uint32_t)e;
+        if ((uint32_t)(e >> 32) == tag && idx < n) {
+            if constexpr (kValue) {
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) vt[kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, idx);
+            }
+            uint64_t c[NP];
+            key_load<NP>(st.key, idx, c);
+            bool eq = true;
+#pragma unroll
+            for (int q = 0; q < NP; ++q) eq = eq && c[q] == kt[q];
+            if (eq) return idx;
+        }
+    }
+}
+
 // Looks up the orbit members T != 0 (valid: subsets of the anticommuting set)
 // among slots [0, n), loads their values and records their replay facts: home
 // index entries for all members first (hashes from the slot's lin_hash lk),
-// then the key checks, then full probes for the rare misses.
+// then the key checks, then the probes past the home entry for the rest.
 template <class Alg, class CP, int K>
 __device__ __forceinline__ void orbit_gather(const Store<CP, K>& st, const typename CP::store_t* cin,
                                              const Window<Alg, CP>& w, const uint64_t (&k)[Alg::NP], uint64_t lk,
@@ -150,9 +185,11 @@ __device__ __forceinline__ void orbit_gather(const Store<CP, K>& st, const typen
                     // the candidate's value travels with its key words (one round less)
 #pragma unroll
                     for (int kk = 0; kk < K; ++kk) v[T][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, idx);
+                    uint64_t kc[NP];
+                    key_load<NP>(st.key, idx, kc);
                     bool eq = true;
 #pragma unroll
-                    for (int q = 0; q < NP; ++q) eq = eq && (st.key[q * cap + idx] == kt[q]);
+                    for (int q = 0; q < NP; ++q) eq = eq && kc[q] == kt[q];
                     if (eq) {
                         slot[T] = idx;
                         early |= 1u << T;
@@ -172,7 +209,8 @@ __device__ __forceinline__ void orbit_gather(const Store<CP, K>& st, const typen
                 if ((need_probe >> T) & 1) {
                     uint64_t kt[NP];
                     orbit_key<T>(k, w, kt);
-                    slot[T] = probe<NP>(st.key, cap, st.index, st.hmask, kt, n, &st.ctrl->index_full);
+                    slot[T] = probe_next<Alg, CP, K, true>(st, cin, kt, hmix(lk ^ lp_of<T>(w)), n, v[T]);
+                    if (slot[T] != kNone) early |= 1u << T;
                 }
             }
         });
@@ -320,15 +358,13 @@ struct AppendSink {
 // Appends the fresh orbit members (absent, live at the end) of the leaders of
 // this block iteration: one atomic per block and iteration.
 template <class Alg, class CP, int K>
-__device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP::store_t* cout,
-                                             const Window<Alg, CP>& w, const AppendSink& sink, uint32_t n,
-                                             const uint64_t (&k)[Alg::NP], uint64_t lk,
-                                             const typename CP::val_t (&v)[kOrbit][K], uint32_t fresh,
-                                             uint32_t* sbase) {
+__device__ __forceinline__ void orbit_append_ranked(const Store<CP, K>& st, typename CP::store_t* cout,
+                                                    const Window<Alg, CP>& w, const AppendSink& sink, uint32_t n,
+                                                    const uint64_t (&k)[Alg::NP], uint64_t lk,
+                                                    const typename CP::val_t (&v)[kOrbit][K], uint32_t fresh,
+                                                    uint32_t rank, uint32_t tot, uint32_t* sbase) {
     constexpr int NP = Alg::NP;
     const uint32_t napp = __popc(fresh);
-    uint32_t tot;
-    const uint32_t rank = block_exclusive_scan(napp, &tot);
     if (tot) {
         if (threadIdx.x == 0) *sbase = atomicAdd(sink.ctr, tot);
         __syncthreads();
@@ -347,8 +383,7 @@ __device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP
                         if (at < cap) {
                             uint64_t kt[NP];
                             orbit_key<T>(k, w, kt);
-#pragma unroll
-                            for (int q = 0; q < NP; ++q) st.key[q * cap + at] = kt[q];
+                            key_store<NP>(st.key, at, kt);
 #pragma unroll
                             for (int kk = 0; kk < K; ++kk)
                                 CP::store(cout + (size_t)kk * CP::NS * cap, cap, at, v[T][kk]);
@@ -386,14 +421,124 @@ __device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP
     }
 }
 
+template <class Alg, class CP, int K>
+__device__ __forceinline__ void orbit_append(const Store<CP, K>& st, typename CP::store_t* cout,
+                                             const Window<Alg, CP>& w, const AppendSink& sink, uint32_t n,
+                                             const uint64_t (&k)[Alg::NP], uint64_t lk,
+                                             const typename CP::val_t (&v)[kOrbit][K], uint32_t fresh,
+                                             uint32_t* sbase) {
+    uint32_t tot;
+    const uint32_t rank = block_exclusive_scan(__popc(fresh), &tot);
+    orbit_append_ranked<Alg, CP, K>(st, cout, w, sink, n, k, lk, v, fresh, rank, tot, sbase);
+}
+
+// A block's appends gathered in LDS over all of its tiles and written out with
+// one bump of the epoch's append counter (a counter every block bumps once per
+// tile serializes: ~11 ns per bump, thousands of tiles per launch).
+template <int NP, class CP, int K>
+struct AppendStage {
+    static constexpr int kEntry = 8 * NP + (int)sizeof(typename CP::val_t) * K;
+    static constexpr int kFit = (24576 / kEntry) & ~63;
+    static constexpr int kCap = kFit < 512 ? kFit : 512;
+    uint64_t key[kCap][NP];
+    typename CP::val_t val[kCap][K];
+    uint32_t used;
+};
+
+// Writes the staged appends to slots [n + base, n + base + used) and indexes
+// them.  Block-uniform.
+template <int NP, class CP, int K>
+__device__ __forceinline__ void stage_flush(const Store<CP, K>& st, typename CP::store_t* cout,
+                                            const AppendSink& sink, uint32_t n, AppendStage<NP, CP, K>& stg,
+                                            uint32_t* sbase) {
+    const uint32_t used = stg.used;
+    if (used == 0) return;
+    if (threadIdx.x == 0) *sbase = atomicAdd(sink.ctr, used);
+    __syncthreads();
+    const uint64_t at0 = (uint64_t)n + *sbase;
+    const size_t cap = st.cap;
+    for (uint32_t r = threadIdx.x; r < used; r += blockDim.x) {
+        const uint64_t at = at0 + r;
+        if (at < cap) {
+            uint64_t kt[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) kt[q] = stg.key[r][q];
+            key_store<NP>(st.key, at, kt);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) CP::store(cout + (size_t)kk * CP::NS * cap, cap, at, stg.val[r][kk]);
+            index_insert(st.index, st.hmask, key_hash<NP>(kt), (uint32_t)at, &st.ctrl->index_full);
+        } else {
+            atomicOr(sink.flag, 1u);
+            if (sink.flag_epoch) *sink.flag_epoch = sink.e;
+        }
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) stg.used = 0;
+    __syncthreads();
+}
+
+// Stages the fresh orbit members of this tile's leaders (flushing first when
+// they do not fit; a tile with more than the stage holds appends directly).
+// Block-uniform.
+template <class Alg, class CP, int K>
+__device__ __forceinline__ void orbit_stage(const Store<CP, K>& st, typename CP::store_t* cout,
+                                            const Window<Alg, CP>& w, const AppendSink& sink, uint32_t n,
+                                            const uint64_t (&k)[Alg::NP], uint64_t lk,
+                                            const typename CP::val_t (&v)[kOrbit][K], uint32_t fresh,
+                                            AppendStage<Alg::NP, CP, K>& stg, uint32_t* sbase) {
+    constexpr int NP = Alg::NP;
+    using Stage = AppendStage<NP, CP, K>;
+    uint32_t tot;
+    const uint32_t rank = block_exclusive_scan(__popc(fresh), &tot);
+    if (tot == 0) return;
+    if (stg.used + tot > (uint32_t)Stage::kCap) {
+        stage_flush<NP, CP, K>(st, cout, sink, n, stg, sbase);
+        if (tot > (uint32_t)Stage::kCap) {
+            orbit_append_ranked<Alg, CP, K>(st, cout, w, sink, n, k, lk, v, fresh, rank, tot, sbase);
+            return;
+        }
+    }
+    const uint32_t u0 = stg.used;
+    if (fresh) {
+        uint32_t r = u0 + rank;
+        sfor<kOrbit>([&](auto TT) {
+            constexpr int T = decltype(TT)::value;
+            if constexpr (T > 0) {
+                if ((fresh >> T) & 1) {
+                    uint64_t kt[NP];
+                    orbit_key<T>(k, w, kt);
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) stg.key[r][q] = kt[q];
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) stg.val[r][kk] = v[T][kk];
+                    ++r;
+                }
+            }
+        });
+    }
+    __syncthreads();
+    if (threadIdx.x == 0) stg.used = u0 + tot;
+    __syncthreads();
+}
+
 // One tile of blockDim slots [b0, b0 + blockDim) of a window: every slot in
 // [0, n) replays its orbit and writes its own value; the leaders append.
 // Block-uniform (contains barriers).
-template <class Alg, class CP, int K>
+// A slot's key and value, loaded ahead by the caller (passed by value).
+template <int NP, class CP, int K>
+struct SlotLoad {
+    uint64_t k[NP];
+    typename CP::val_t v[K];
+};
+
+// pre: the slot's key and value are already in `ld` (loaded by the caller
+// together with other independent loads).
+template <class Alg, class CP, int K, bool kStaged = false>
 __device__ __forceinline__ void win_tile(const Store<CP, K>& st, const typename CP::store_t* cin,
                                          typename CP::store_t* cout, const Window<Alg, CP>& w, uint32_t n,
                                          uint32_t b0, const AppendSink& sink, uint32_t* sbase, int& dd,
-                                         int (&sdl)[kWin]) {
+                                         int (&sdl)[kWin], AppendStage<Alg::NP, CP, K>* stg, bool pre,
+                                         SlotLoad<Alg::NP, CP, K> ld) {
     constexpr int NP = Alg::NP;
     using val_t = typename CP::val_t;
     const size_t cap = st.cap;
@@ -403,10 +548,16 @@ __device__ __forceinline__ void win_tile(const Store<CP, K>& st, const typename 
     uint64_t lk = 0;
     val_t v[kOrbit][K];
     if (i < n) {
+        if (pre) {
 #pragma unroll
-        for (int q = 0; q < NP; ++q) k[q] = st.key[q * cap + i];
+            for (int q = 0; q < NP; ++q) k[q] = ld.k[q];
 #pragma unroll
-        for (int kk = 0; kk < K; ++kk) v[0][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, i);
+            for (int kk = 0; kk < K; ++kk) v[0][kk] = ld.v[kk];
+        } else {
+            key_load<NP>(st.key, i, k);
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) v[0][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, i);
+        }
         const uint32_t amask = window_amask<Alg>(k, w.g, w.k);
         const uint32_t valid = subsets_of<Alg>(amask);
         uint32_t alive = CP::dead(v[0][0]) ? 0u : 1u, exist = 1;
@@ -431,7 +582,10 @@ __device__ __forceinline__ void win_tile(const Store<CP, K>& st, const typename 
         dd += (int)a0 - (int)(alive & 1u);
         if (leader) fresh = alive & ~exist;
     }
-    orbit_append<Alg, CP, K>(st, cout, w, sink, n, k, lk, v, fresh, sbase);
+    if constexpr (kStaged)
+        orbit_stage<Alg, CP, K>(st, cout, w, sink, n, k, lk, v, fresh, *stg, sbase);
+    else
+        orbit_append<Alg, CP, K>(st, cout, w, sink, n, k, lk, v, fresh, sbase);
 }
 
 __device__ __forceinline__ void window_report(const Epochs& ep, uint32_t e, int dd, const int (&sdl)[kWin]) {
@@ -439,9 +593,10 @@ __device__ __forceinline__ void window_report(const Epochs& ep, uint32_t e, int 
     block_sum_ints<1 + kWin>(r);
     if (threadIdx.x == 0) {
         if (r[0]) atomicAdd(&ep.dd[e], r[0]);
+        int32_t* sd = sd_stripe(ep, blockIdx.x);
 #pragma unroll
         for (int s = 0; s < kWin; ++s)
-            if (r[1 + s]) atomicAdd(&ep.sd[(size_t)kWin * e + s], r[1 + s]);
+            if (r[1 + s]) atomicAdd(&sd[(size_t)kWin * e + s], r[1 + s]);
     }
 }
 
@@ -451,16 +606,33 @@ template <class Alg, class CP, int K>
 __global__ void __launch_bounds__(kBlockWin) k_win_async(Store<CP, K> st, const typename CP::store_t* cin,
                                                          typename CP::store_t* cout, Window<Alg, CP> w, Epochs ep,
                                                          uint32_t e) {
+    // block-strided tiles of blockDim slots; the first tile's key and value are
+    // loaded together with the epoch counters (slot < cap, so in bounds whatever
+    // n turns out to be; unused beyond n).  They travel to the tile by value:
+    // a reference to a local array here made the gfx950 backend emit a
+    // private-address check it then rejects ("operand has incorrect register class").
+    constexpr bool kPre = true;
+    const uint32_t begin = blockIdx.x * blockDim.x, step = gridDim.x * blockDim.x;
+    const uint32_t i0 = begin + threadIdx.x;
+    SlotLoad<Alg::NP, CP, K> ld0{};
+    if (kPre && i0 < st.cap) {
+        key_load<Alg::NP>(st.key, i0, ld0.k);
+#pragma unroll
+        for (int kk = 0; kk < K; ++kk) ld0.v[kk] = CP::load(cin + (size_t)kk * CP::NS * st.cap, st.cap, i0);
+    }
     uint32_t n, dead;
     if (!step_enter(ep, e, n, dead)) return;
-    // block-strided tiles of blockDim slots
-    const uint32_t begin = blockIdx.x * blockDim.x, step = gridDim.x * blockDim.x;
     if (begin >= n) return;  // block-uniform: nothing to do, nothing to report
     __shared__ uint32_t sbase;
+    __shared__ AppendStage<Alg::NP, CP, K> stg;
+    if (threadIdx.x == 0) stg.used = 0;
+    __syncthreads();
     const AppendSink sink{&ep.app[e], &ep.flags[0], &ep.flags[1], e};
     int dd = 0;
     int sdl[kWin] = {0, 0, 0, 0};
-    for (uint32_t b0 = begin; b0 < n; b0 += step) win_tile<Alg, CP, K>(st, cin, cout, w, n, b0, sink, &sbase, dd, sdl);
+    for (uint32_t b0 = begin; b0 < n; b0 += step)
+        win_tile<Alg, CP, K, true>(st, cin, cout, w, n, b0, sink, &sbase, dd, sdl, &stg, kPre && b0 == begin, ld0);
+    stage_flush<Alg::NP, CP, K>(st, cout, sink, n, stg, &sbase);
     window_report(ep, e, dd, sdl);
 }
 
