@@ -40,6 +40,25 @@ struct Epochs {
     uint32_t sd_stride;
 };
 
+// Sums the kStripes copies of striped counters (stride words apart): out32[j]
+// for j < len, and the 64-bit word at kWin * q of every copy into out64[q] for
+// q < ne (the run-survival counts of Clifford-rotation runs).
+static __global__ void k_sum_stripes(const int32_t* sd, uint32_t stride, uint32_t len, uint32_t ne,
+                                     int32_t* out32, unsigned long long* out64, int win) {
+    const uint32_t t0 = blockIdx.x * blockDim.x + threadIdx.x, nt = gridDim.x * blockDim.x;
+    for (uint32_t j = t0; j < len; j += nt) {
+        int32_t a = 0;
+        for (int c = 0; c < kStripes; ++c) a += sd[(size_t)c * stride + j];
+        out32[j] = a;
+    }
+    for (uint32_t q = t0; q < ne; q += nt) {
+        unsigned long long a = 0;
+        for (int c = 0; c < kStripes; ++c)
+            a += *(const unsigned long long*)(sd + (size_t)c * stride + (size_t)win * q);
+        out64[q] = a;
+    }
+}
+
 // This block's copy of the striped per-rotation counters.
 __device__ __forceinline__ int32_t* sd_stripe(const Epochs& ep, uint32_t copy) {
     return ep.sd + (size_t)(copy % kStripes) * ep.sd_stride;

@@ -619,8 +619,13 @@ struct Engine {
     }
 
     // n, dead, app, dd (one word per epoch each), flags (64 words), sd (kWin per epoch,
-    // kStripes copies)
-    static size_t epoch_words(size_t c) { return (4 + (size_t)kStripes * kWin) * c + 64; }
+    // kStripes copies), then the sums of the copies: kWin words and one 64-bit word
+    // per epoch
+    static size_t epoch_words(size_t c) { return (4 + (size_t)kStripes * kWin) * c + 64 + (kWin + 2) * c; }
+    int32_t* sd_sum32() const { return (int32_t*)ctx->epochs + (4 + (size_t)kStripes * kWin) * ctx->epochs_cap + 64; }
+    unsigned long long* sd_sum64() const {
+        return (unsigned long long*)(ctx->epochs + (4 + (size_t)kStripes * kWin + kWin) * ctx->epochs_cap + 64);
+    }
 
     Epochs epochs() const {
         Epochs ep;
@@ -827,7 +832,7 @@ struct Engine {
         const size_t lpos_off = (lrec_bytes + 255) & ~(size_t)255;
         const size_t lsd_off = lpos_off + ((n_layers * 256 * sizeof(int16_t) + 255) & ~(size_t)255);
         const size_t lsd_stride = (n_layer_rot + 63) & ~(size_t)63;  // striped counter copies
-        layer_bytes = lsd_off + kStripes * lsd_stride * sizeof(int32_t);
+        layer_bytes = lsd_off + (kStripes + 1) * lsd_stride * sizeof(int32_t);  // copies, then their sums
         constexpr size_t kLM = 3 * FFX_MAX_WORDS + 8;  // per layer: mx, mz, my (4 words each), tmask, xmask
         std::vector<uint64_t> layer_masks;
         if (n_layers) {
@@ -1088,29 +1093,34 @@ struct Engine {
             ctx->prof.seconds_enqueue += now_s() - te0;
             // one sync per run: epochs, flags
             std::vector<uint32_t> hn(e), hd(e), ha(e);
-            std::vector<int32_t> hdd(e), hsd_all((size_t)kStripes * kWin * e);
+            std::vector<int32_t> hdd(e), hsd((size_t)kWin * e), hlsd(n_layer_rot);
+            std::vector<unsigned long long> hsurv(e);
             uint32_t hf[2] = {0, 0};
             HIPCHK(hipMemcpyAsync(hn.data(), ep.n, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(hd.data(), ep.dead, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(ha.data(), ep.app, e * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(hdd.data(), ep.dd, e * sizeof(int32_t), hipMemcpyDeviceToHost, ctx->stream));
             HIPCHK(hipMemcpyAsync(hf, ep.flags, 2 * sizeof(uint32_t), hipMemcpyDeviceToHost, ctx->stream));
-            HIPCHK(hipMemcpy2DAsync(hsd_all.data(), (size_t)kWin * e * sizeof(int32_t), ep.sd,
-                                    (size_t)ep.sd_stride * sizeof(int32_t), (size_t)kWin * e * sizeof(int32_t),
-                                    kStripes, hipMemcpyDeviceToHost, ctx->stream));
-            std::vector<int32_t> hlsd_all((size_t)kStripes * n_layer_rot);
-            if (n_layer_rot)
-                HIPCHK(hipMemcpy2DAsync(hlsd_all.data(), n_layer_rot * sizeof(int32_t),
-                                        (char*)ctx->layer_buf + lsd_off, lsd_stride * sizeof(int32_t),
-                                        n_layer_rot * sizeof(int32_t), kStripes, hipMemcpyDeviceToHost,
-                                        ctx->stream));
-            HIPCHK(hipStreamSynchronize(ctx->stream));
-            // sums of the striped counter copies
-            std::vector<int32_t> hsd((size_t)kWin * e, 0), hlsd(n_layer_rot, 0);
-            for (int c = 0; c < kStripes; ++c) {
-                for (size_t j = 0; j < hsd.size(); ++j) hsd[j] += hsd_all[(size_t)c * kWin * e + j];
-                for (size_t j = 0; j < n_layer_rot; ++j) hlsd[j] += hlsd_all[(size_t)c * n_layer_rot + j];
+            // the striped counter copies, summed on the device
+            hipLaunchKernelGGL(k_sum_stripes, dim3(grid_for((size_t)kWin * e, kBlock, 64)), dim3(kBlock), 0,
+                               ctx->stream, (const int32_t*)ep.sd, ep.sd_stride, (uint32_t)(kWin * e), (uint32_t)e,
+                               sd_sum32(), sd_sum64(), kWin);
+            HIPCHK(hipGetLastError());
+            HIPCHK(hipMemcpyAsync(hsd.data(), sd_sum32(), hsd.size() * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            HIPCHK(hipMemcpyAsync(hsurv.data(), sd_sum64(), e * sizeof(unsigned long long), hipMemcpyDeviceToHost,
+                                  ctx->stream));
+            if (n_layer_rot) {
+                int32_t* lsum = (int32_t*)((char*)ctx->layer_buf + lsd_off) + kStripes * lsd_stride;
+                hipLaunchKernelGGL(k_sum_stripes, dim3(grid_for(n_layer_rot, kBlock, 64)), dim3(kBlock), 0,
+                                   ctx->stream, (const int32_t*)((char*)ctx->layer_buf + lsd_off),
+                                   (uint32_t)lsd_stride, (uint32_t)n_layer_rot, 0u, lsum,
+                                   (unsigned long long*)nullptr, kWin);
+                HIPCHK(hipGetLastError());
+                HIPCHK(hipMemcpyAsync(hlsd.data(), lsum, n_layer_rot * sizeof(int32_t), hipMemcpyDeviceToHost,
+                                      ctx->stream));
             }
+            HIPCHK(hipStreamSynchronize(ctx->stream));
             for (size_t q = 0; q < nev; ++q) {
                 float ms = 0.f;
                 HIPCHK(hipEventElapsedTime(&ms, ctx->kev[2 * q], ctx->kev[2 * q + 1]));
@@ -1123,12 +1133,7 @@ struct Engine {
                     // a run of Clifford rotations: the kernel sums, over the terms live at its
                     // start, the rotations each one survives; a term erased inside the run also
                     // entered the rotation that erased it
-                    uint64_t surv = 0;
-                    for (int c = 0; c < kStripes; ++c) {
-                        uint64_t part = 0;
-                        std::memcpy(&part, &hsd_all[(size_t)c * kWin * e + (size_t)kWin * q], sizeof(part));
-                        surv += part;
-                    }
+                    const uint64_t surv = hsurv[q];
                     const int64_t lout = (int64_t)(hn[q] + ha[q]) - (int64_t)((int32_t)hd[q] + hdd[q]);
                     st_rot += crot_len[q];
                     st_updates += (int64_t)surv + hdd[q];
