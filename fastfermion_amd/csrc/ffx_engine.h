@@ -602,6 +602,7 @@ struct Engine {
         int topk_xw = 0;        // ... followed by the X/Y-weight cap
         int cb = 0, cn = 0;     // Clifford gates [cb, cb+cn) of the block list
         int rb = 0, rn = 0;     // kind 2: Clifford rotations [rb, rb+rn) of the record list
+        int rcomm = 0;          // kind 2: their axes commute pairwise
         int prune_after = 0;    // learned pruning after this step
         FeatCtx fctx{};         // feature context of that pruning
     };
@@ -1015,7 +1016,8 @@ struct Engine {
                     ++e;
                 } else if (a.kind == 2) {
                     hipLaunchKernelGGL((k_crot_async<Alg, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream, store(),
-                                       (const CRot<Alg, CP>*)ctx->crot + a.rb, a.rn, mincoeff, maxdeg, max_xw, ep, e);
+                                       (const CRot<Alg, CP>*)ctx->crot + a.rb, a.rn, a.rcomm, mincoeff, maxdeg, max_xw,
+                                       ep, e);
                     CHK(launch_check());
                     hipLaunchKernelGGL(k_index_build_async<NP>, dim3(g_idx), dim3(kBlock), 0, ctx->stream, ctx->key,
                                        ctx->cap, ctx->index, ctx->hslots - 1, ep, e, e, 0, &ctx->ctrl->index_full);
@@ -1359,6 +1361,41 @@ void prune_plan(std::vector<typename Engine<Alg, CP, K>::AStep>& sd, std::vector
             out.back().fctx = fp.at(done);
         }
     }
+    // per run: the next truncation points of every record, and whether the axes
+    // commute pairwise (all Z-type or all X-type Pauli axes; short runs are
+    // checked pair by pair)
+    for (AStep& a : out) {
+        if (a.kind != 2) continue;
+        CRot<Alg, CP>* r = crots.data() + a.rb;
+        int nth = a.rn, nxw = a.rn;
+        for (int t = a.rn - 1; t >= 0; --t) {
+            if (r[t].thresh) nth = t;
+            if (r[t].xw) nxw = t;
+            r[t].next_th = nth;
+            r[t].next_xw = nxw;
+        }
+        bool comm = false;
+        if constexpr (Alg::kPauli) {
+            constexpr int W = Alg::W;
+            uint64_t xo = 0, zo = 0;
+            for (int t = 0; t < a.rn; ++t)
+                for (int w = 0; w < W; ++w) {
+                    xo |= r[t].g.p[w];
+                    zo |= r[t].g.p[W + w];
+                }
+            comm = xo == 0 || zo == 0;
+        }
+        if (!comm && a.rn <= 48) {
+            comm = true;
+            for (int i = 0; i < a.rn && comm; ++i)
+                for (int j = i + 1; j < a.rn && comm; ++j)
+                    comm = Engine<Alg, CP, K>::axes_commute(r[i].g, r[j].g);
+        }
+        a.rcomm = comm ? 1 : 0;
+    }
+    if (const char* ev = getenv("FFX_CROT_SERIAL"))  // rotation-by-rotation loop for every run (cross-check)
+        if (atoi(ev) > 0)
+            for (AStep& a : out) a.rcomm = 0;
     sd.swap(out);
 }
 

@@ -39,6 +39,8 @@ struct CRot {
     typename CP::fac_t sn;  // sin(theta) (|cos theta| <= tol)
     int thresh;             // threshold after this rotation
     int xw;                 // X/Y-weight cap after this rotation
+    int next_th;            // first rotation of the run at or after this one with a threshold (rn: none)
+    int next_xw;            // same for the X/Y-weight cap
 };
 
 constexpr int kCRotLds = 256;  // Clifford-rotation records staged in LDS per pass
@@ -47,10 +49,67 @@ constexpr int kCRotLds = 256;  // Clifford-rotation records staged in LDS per pa
 // bijection on keys, so keys stay distinct), coefficients in place; the index
 // is cleared and rebuilt by k_index_build_async afterwards.  The records of a
 // run are read from LDS (one staging pass for runs of up to kCRotLds rotations).
+//
+// commuting: the axes of the run commute pairwise.  Then XORing one axis into a
+// key does not change its commutation with another, so the rotations acting on
+// a term are fixed by its key at the start of the run: they are found with
+// independent tests (no dependent chain through the records), only they update
+// the term, and between two of them coefficient and key are constant, so the
+// truncation points there reduce to one test each (next_th / next_xw).  The
+// result is the one of the rotation-by-rotation loop.
+template <class Alg, class CP, int K>
+__device__ __forceinline__ void crot_commuting(const CRot<Alg, CP>* rr, int nr, double mincoeff, int maxdeg,
+                                               int max_xw, uint64_t (&k)[Alg::NP], typename CP::val_t (&c)[K],
+                                               bool alive, int& kill_at) {
+    constexpr int NP = Alg::NP;
+    uint64_t k0[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) k0[q] = k[q];
+    kill_at = nr;
+    // truncation points in [a, b) with the current coefficient and key
+    auto check = [&](int a, int b) {
+        if (!alive || kill_at < nr || a >= b) return;
+        const int th = rr[a].next_th, xp = rr[a].next_xw;
+        int at = nr;
+        if (th < b && magnitude<CP, K>(c) <= mincoeff) at = th;
+        if (xp < b && xp < at && Alg::xweight(k) > max_xw) at = xp;
+        kill_at = at;
+    };
+    int prev = 0;
+    for (int c0 = 0; c0 < nr; c0 += 64) {
+        const int cn = min(64, nr - c0);
+        uint64_t m = 0;
+        for (int j = 0; j < cn; ++j)
+            if (Alg::anticommutes(k0, rr[c0 + j].g)) m |= 1ull << j;
+        while (m) {
+            const int t = c0 + __ffsll((long long)m) - 1;
+            m &= m - 1;
+            check(prev, t);
+            const CRot<Alg, CP>& cr = rr[t];
+            uint64_t r[NP];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) r[q] = k[q] ^ cr.g.p[q];
+            if (alive && kill_at == nr) {
+                if (Alg::degree(r) > maxdeg) {
+                    kill_at = t;
+                } else {
+                    const int sg = Alg::sign(k, r, cr.g);
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) c[kk] = CP::mul_signed(c[kk], cr.sn, sg);
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < NP; ++q) k[q] = r[q];
+            prev = t;
+        }
+    }
+    check(prev, nr);
+}
+
 template <class Alg, class CP, int K>
 __global__ void __launch_bounds__(kBlock) k_crot_async(Store<CP, K> st, const CRot<Alg, CP>* __restrict__ rots,
-                                                       int nr, double mincoeff, int maxdeg, int max_xw, Epochs ep,
-                                                       uint32_t e) {
+                                                       int nr, int commuting, double mincoeff, int maxdeg, int max_xw,
+                                                       Epochs ep, uint32_t e) {
     constexpr int NP = Alg::NP;
     using val_t = typename CP::val_t;
     __shared__ CRot<Alg, CP> srot[kCRotLds];
@@ -73,7 +132,13 @@ __global__ void __launch_bounds__(kBlock) k_crot_async(Store<CP, K> st, const CR
         for (int kk = 0; kk < K; ++kk) c[kk] = st.ld(st.coef, kk, i);
         const bool alive = !CP::dead(c[0]);
         bool kill = false;
-        for (int t = 0; t < nr; ++t) {
+        if (commuting) {
+            int kill_at;
+            crot_commuting<Alg, CP, K>(rr, nr, mincoeff, maxdeg, max_xw, k, c, alive, kill_at);
+            kill = kill_at < nr;
+            if (alive) surv += (unsigned long long)kill_at;
+        }
+        for (int t = 0; t < (commuting ? 0 : nr); ++t) {
             const CRot<Alg, CP>& cr = rr[t];
             if (Alg::anticommutes(k, cr.g)) {
                 uint64_t r[NP];

@@ -218,6 +218,52 @@ def test_clifford_rotations_match_plain(ctx, batched, prec):
         assert upd1 == upd0
 
 
+def xx_clifford_case(rng, nq, layers):
+    """Runs of X-type Clifford rotations (XX(+-pi/2), X(pi/2)) between layers of
+    generic single-qubit Z/Y rotations: every run's axes commute pairwise."""
+    spec = []
+    for _ in range(layers):
+        for _ in range(int(rng.integers(6, 14))):
+            a, b = sorted(rng.choice(nq, 2, replace=False).tolist())
+            ax = "XX" if rng.uniform() < 0.8 else "X"
+            qs = (a, b) if ax == "XX" else (a,)
+            spec.append((ax, qs, float(rng.choice([math.pi / 2, -math.pi / 2]))))
+        for q in rng.choice(nq, int(rng.integers(3, 8)), replace=False).tolist():
+            spec.append((str(rng.choice(["Z", "Y"])), (q,), float(rng.uniform(-1.0, 1.0))))
+    return wl._pack(spec)
+
+
+@pytest.mark.parametrize("case", ["ising_pergate", "ising_batched", "ising_xwcap", "xx_runs"])
+def test_commuting_runs_match_serial_loop(ctx, monkeypatch, case):
+    """A run of pairwise-commuting Clifford rotations is applied from the set of
+    rotations acting on each term (fixed by its key at the start of the run);
+    keys, coefficients and work counters must equal the rotation-by-rotation loop."""
+    kw = dict(maxdegree=7, mincoeff=1e-5, batched=case == "ising_batched")
+    if case.startswith("ising"):
+        gates, W, x, z, c, po = ising_case(5, 6)
+        if case == "ising_xwcap":
+            kw.update(max_xweight=3, xtrunc_period=4)
+    else:
+        rng = np.random.default_rng(21)
+        nq = 40
+        gates = xx_clifford_case(rng, nq, 8)
+        W = 1
+        x, z, c = wl.single_pauli("Z", [7], nq)
+        x2, z2, c2 = wl.single_pauli("Y", [11], nq)
+        x, z, c = np.concatenate([x, x2]), np.concatenate([z, z2]), np.concatenate([c, 0.5 * c2])
+        row, col = [0] * nq, list(range(nq))
+        po = prune_opts(row, col, 7, rot_per_layer=16)
+    po.clifford_tol = 1e-12
+    res = {}
+    for serial in ("1", "0"):
+        monkeypatch.setenv("FFX_CROT_SERIAL", serial)
+        r = ctx.pauli_propagate_pruned(gates, W, x, z, c, po, **kw)
+        res[serial] = (r, ctx.profile())
+    assert_same_terms(res["0"][0], res["1"][0], exact=True, what=f"commuting runs {case}")
+    for k in ("updates", "outs", "peak_terms"):
+        assert res["0"][1][k] == res["1"][1][k], (k, res["0"][1][k], res["1"][1][k])
+
+
 def test_clifford_rotations_majorana(ctx):
     rng = np.random.default_rng(8)
     nm = 12
