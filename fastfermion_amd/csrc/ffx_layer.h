@@ -154,9 +154,15 @@ __global__ void __launch_bounds__(kBlock) k_layer_async(Store<CP, K> st, const t
     __shared__ uint32_t scount[kLayerMaxA + 1];
     uint32_t n, dead;
     if (!step_enter(ep, e, n, dead)) return;
-    // slots per block iteration: 64 when every slot fits the grid that way (small term
-    // sets: four waves share the orbits of one wave's slots), else one per thread
-    const uint32_t tile = (uint64_t)n <= (uint64_t)64 * gridDim.x ? 64u : blockDim.x;
+    // slots per block iteration: the fewest of 4, 16, 64 that fit every slot into the
+    // grid (small term sets: a block's waves share the orbits of a few slots, and the
+    // orbits spread over many blocks), else one per thread
+    uint32_t tile = blockDim.x;
+    for (uint32_t t = 4; t < blockDim.x; t <<= 2)
+        if ((uint64_t)n <= (uint64_t)t * gridDim.x) {
+            tile = t;
+            break;
+        }
     const uint32_t step = gridDim.x * tile;
     if (blockIdx.x * tile >= n) return;  // block-uniform
     for (int t = threadIdx.x; t < 256; t += blockDim.x) spos[t] = L.pos[t];
@@ -275,11 +281,15 @@ __global__ void __launch_bounds__(kBlock) k_layer_async(Store<CP, K> st, const t
         }
         __syncthreads();
         // ---- phase 2a: orbits with <= 6 active qubits, 64 / 2^A of them per wave pass ----
-        // (lane group g = lane >> A holds one orbit, member T = lane & (2^A - 1))
+        // (lane group g = lane >> A holds one orbit, member T = lane & (2^A - 1)); the
+        // passes of all A are numbered in one sequence that the waves take round robin
+        int poff = 0;
         for (int A = 1; A <= 6; ++A) {
             const int nlead = (int)scount[A], per = 64 >> A;
             const int npass = (nlead + per - 1) / per;
-            for (int pass = wave; pass < npass; pass += nwave) {
+            const int pass0 = ((wave - poff) % nwave + nwave) % nwave;
+            poff += npass;
+            for (int pass = pass0; pass < npass; pass += nwave) {
                 const int g = lane >> A, T = lane & ((1 << A) - 1);
                 const int kth = pass * per + g;
                 const bool on = kth < nlead;
@@ -428,7 +438,7 @@ __global__ void __launch_bounds__(kBlock) k_layer_async(Store<CP, K> st, const t
         }
         // ---- phase 2b: orbits with 7 or 8 active qubits, one per wave pass ----
         const int n7 = (int)scount[7], n78 = n7 + (int)scount[8];
-        for (int jb = wave; jb < n78; jb += nwave) {
+        for (int jb = ((wave - poff) % nwave + nwave) % nwave; jb < n78; jb += nwave) {
             const int src = jb < n7 ? (int)slist[7][jb] : (int)slist[8][jb - n7];
             uint64_t lk[NP], lam[W];
 #pragma unroll
