@@ -606,14 +606,18 @@ template <class Alg, class CP, int K>
 __global__ void __launch_bounds__(kBlockWin) k_win_async(Store<CP, K> st, const typename CP::store_t* cin,
                                                          typename CP::store_t* cout, Window<Alg, CP> w, Epochs ep,
                                                          uint32_t e) {
-    // block-strided tiles of blockDim slots; the first tile's key and value are
-    // loaded together with the epoch counters (slot < cap, so in bounds whatever
-    // n turns out to be; unused beyond n).  They travel to the tile by value:
-    // a reference to a local array here made the gfx950 backend emit a
-    // private-address check it then rejects ("operand has incorrect register class").
-    constexpr bool kPre = true;
+    // block-strided tiles of blockDim slots; in the first kPreBlocks blocks the
+    // first tile's key and value are loaded together with the epoch counters
+    // (slot < cap, so in bounds whatever n turns out to be; unused beyond n).
+    // Small windows, whose launches are a chain of dependent loads, gain a round
+    // trip; the bound keeps the unused loads of the blocks past n to ~1 MB.  The
+    // values travel to the tile by value: a reference to a local array here made
+    // the gfx950 backend emit a private-address check it then rejects ("operand has
+    // incorrect register class").
+    constexpr uint32_t kPreBlocks = 128;
     const uint32_t begin = blockIdx.x * blockDim.x, step = gridDim.x * blockDim.x;
     const uint32_t i0 = begin + threadIdx.x;
+    const bool kPre = blockIdx.x < kPreBlocks;
     SlotLoad<Alg::NP, CP, K> ld0{};
     if (kPre && i0 < st.cap) {
         key_load<Alg::NP>(st.key, i0, ld0.k);
