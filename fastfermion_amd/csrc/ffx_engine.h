@@ -218,19 +218,46 @@ std::vector<Step> pauli_schedule(const ffx_pauli_gate* g, int64_t n, bool batche
     bool pending = false;
     int64_t cend = 0;
     std::vector<int64_t> buf;
+    // rotations of the growing batch by qubit: a rotation can only fail to commute
+    // with batch members that share a qubit with it, so it is checked against those
+    // (the same batches as checking every member, without the quadratic cost)
+    std::vector<std::vector<int32_t>> on_q(64 * W);
+    std::vector<int> touched;
+    auto support = [&](const ffx_pauli_gate& r, auto&& f) {
+        for (int w = 0; w < W; ++w) {
+            uint64_t m = r.x[w] | r.z[w];
+            while (m) {
+                f(64 * w + __builtin_ctzll(m));
+                m &= m - 1;
+            }
+        }
+    };
     auto flush = [&]() {
         if (buf.empty()) return;
         size_t s = 0;
         while (s < buf.size()) {
+            for (int q : touched) on_q[q].clear();
+            touched.clear();
+            auto add = [&](size_t t) {
+                support(g[buf[t]], [&](int q) {
+                    if (on_q[q].empty()) touched.push_back(q);
+                    on_q[q].push_back((int32_t)t);
+                });
+            };
+            add(s);
             size_t e = s + 1;
             while (e < buf.size()) {
                 bool ok = true;
-                for (size_t t = s; t < e; ++t)
-                    if (!rot_commute<W>(g[buf[e]], g[buf[t]])) {
-                        ok = false;
-                        break;
-                    }
+                support(g[buf[e]], [&](int q) {
+                    if (!ok) return;
+                    for (int32_t t : on_q[q])
+                        if (!rot_commute<W>(g[buf[e]], g[buf[t]])) {
+                            ok = false;
+                            return;
+                        }
+                });
                 if (!ok) break;
+                add(e);
                 ++e;
             }
             for (size_t t = s; t < e; ++t) {
