@@ -177,3 +177,47 @@ def test_majorana_windows_bit_exact(ctx, monkeypatch, nm):
         check_same(res, f"majorana nm={nm}")
         ref = pyffo.majorana_propagate(g, W, m, c, maxdegree=maxdeg, mincoeff=minc)
         assert_same_terms(res[-1][0], ref, exact=True, what=f"nm={nm} maxdeg={maxdeg}")
+
+
+@pytest.mark.parametrize("grid", ["0", "1"])
+@pytest.mark.parametrize("every", [1, 16])
+def test_staged_appends_overflow_and_flush(ctx, monkeypatch, grid, every):
+    """Appends of a window launch are staged in LDS per block and written out
+    with one counter bump.  Terms X0 X1 X2 X3 (x) (distinct Z patterns on qubits
+    4..14) have 15 absent orbit members each under Z rotations on qubits 0..3
+    (one window of 4); the other terms carry Z0 and stay out of every orbit.
+    every=1: each tile has ~3800 fresh members, more than the stage holds, and
+    appends directly; every=16: ~240 per tile are staged, and a block that walks
+    several tiles (a one-block grid) flushes its stage on the way."""
+    n_terms = 300 if every == 1 else 1200
+    xs, zs = [], []
+    for p in range(1, n_terms + 1):
+        bits = [b for b in range(11) if (p >> b) & 1]
+        if p % every == 0:
+            ops, qs = "XXXX" + "Z" * len(bits), [0, 1, 2, 3] + [4 + b for b in bits]
+        else:
+            ops, qs = "Z" + "Z" * len(bits), [0] + [4 + b for b in bits]
+        x, z = _abi.pauli_word(ops, qs, 1)
+        xs.append(x)
+        zs.append(z)
+    x = np.array(xs, np.uint64).reshape(-1, 1)
+    z = np.array(zs, np.uint64).reshape(-1, 1)
+    rng = np.random.default_rng(5)
+    c = rng.normal(size=n_terms).astype(np.complex128)
+    gates = wl._pack([("Z", (q,), float(rng.uniform(-1.2, 1.2))) for q in range(4)] * 3)
+    kw = dict(maxdegree=16, mincoeff=0.0, batched=True)
+    monkeypatch.setenv("FFX_WIN_GRID", grid)
+    got = ctx.pauli_propagate(gates, 1, x, z, c, **kw)
+    pg = ctx.profile()
+    monkeypatch.setenv("FFX_WINDOW", "1")
+    ref = ctx.pauli_propagate(gates, 1, x, z, c, **kw)
+    pr = ctx.profile()
+    monkeypatch.delenv("FFX_WINDOW")
+    monkeypatch.delenv("FFX_WIN_GRID")
+    active = n_terms // every
+    assert len(got) == n_terms + 15 * active
+    assert_same_terms(got, ref, exact=True, what=f"staged appends grid={grid} every={every}")
+    for k in ("updates", "outs", "peak_terms"):
+        assert pg[k] == pr[k], (k, pg[k], pr[k])
+    orc = pyffo.pauli_propagate(gates, 1, x, z, c, **kw)
+    assert_same_terms(got, orc, exact=True, what="staged appends vs oracle")
