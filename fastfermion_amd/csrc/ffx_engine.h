@@ -197,6 +197,18 @@ struct Step {
     int64_t cb = 0, ce = 0;  // Clifford block [cb, ce)
 };
 
+// With truncation after every rotation, the device-decided compaction sequence
+// (five launches that return at once when a quarter of the slots is not erased)
+// is enqueued after every compact_every()-th truncation point (FFX_COMPACT_EVERY,
+// default 64): at 16, the no-op sequences cost ~10 of the 35 ms of the 6x6
+// Fermi-Hubbard propagation.
+inline int compact_every() {
+    int k = 64;
+    if (const char* ev = getenv("FFX_COMPACT_EVERY"))
+        if (atoi(ev) > 0) k = atoi(ev);
+    return k;
+}
+
 template <int W>
 bool rot_commute(const ffx_pauli_gate& a, const ffx_pauli_gate& b) {
     int p1 = 0, p2 = 0;
@@ -1343,7 +1355,7 @@ void prune_plan(std::vector<typename Engine<Alg, CP, K>::AStep>& sd, std::vector
     std::vector<AStep> out;
     out.reserve(sd.size());
     size_t done = 0, compacts = 0;
-    size_t thin = 4;  // keep every thin-th compaction point
+    size_t thin = 16;  // keep every thin-th compaction point
     if (const char* ev = getenv("FFX_PRUNE_COMPACT"))
         if (atoi(ev) > 0) thin = (size_t)atoi(ev);
     for (AStep a : sd) {
@@ -1352,7 +1364,7 @@ void prune_plan(std::vector<typename Engine<Alg, CP, K>::AStep>& sd, std::vector
             continue;
         }
         ++done;
-        // pruned runs keep few terms: a quarter of the compaction points is enough
+        // pruned runs keep few terms: a sixteenth of the compaction points is enough
         if (a.compact_after) a.compact_after = (compacts++ % thin) == thin - 1 ? 1 : 0;
         const bool cliff = po.clifford_tol > 0 && std::fabs(a.cs) <= po.clifford_tol;
         if (cliff) {
@@ -1434,6 +1446,7 @@ void pauli_async_steps(const ffx_pauli_gate* gates, const std::vector<Step>& ste
     using AStep = typename Engine<PauliAlg<W>, CP, K>::AStep;
     sd.reserve(steps.size());
     int trunc_points = 0;
+    const int every = compact_every();
     for (const Step& s : steps) {
         AStep d{};
         if (s.kind == 0) {
@@ -1453,7 +1466,7 @@ void pauli_async_steps(const ffx_pauli_gate* gates, const std::vector<Step>& ste
                 d.trunc_xw = 0;
             }
             // batched runs compact at every batch end; per-gate runs every 16th truncation
-            if (s.trunc) d.compact_after = (o.batched || (++trunc_points % 16) == 0) ? 1 : 0;
+            if (s.trunc) d.compact_after = (o.batched || (++trunc_points % every) == 0) ? 1 : 0;
         } else {
             d.kind = 1;
             d.cb = (int)cl.size();
@@ -1555,13 +1568,14 @@ int run_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, 
         std::vector<AStep> sd;
         sd.reserve((size_t)n_gates);
         int cnt = 0;
+        const int every = compact_every();
         for (int64_t i = n_gates - 1; i >= 0; --i) {
             AStep d{};
             d.g = majorana_gate_dev<W>(gates[i]);
             gate_trig(gates[i].theta, d.cs, d.sn);
             d.kind = 0;
             d.trunc_thresh = mincoeff > 0 ? 1 : 0;
-            d.compact_after = (mincoeff > 0 && (++cnt % 16) == 0) ? 1 : 0;
+            d.compact_after = (mincoeff > 0 && (++cnt % every) == 0) ? 1 : 0;
             sd.push_back(d);
         }
         std::vector<CRot<Alg, CP>> crots;
