@@ -68,7 +68,8 @@ struct ffx_ctx {
     uint64_t* index = nullptr;
     size_t cap = 0;      // slots
     size_t hslots = 0;   // index entries (power of two, >= 2*cap)
-    size_t key_planes = 0, coef_planes = 0, coef_elem = 0;  // layout of the current store
+    size_t key_planes = 0, coef_planes = 0, coef_elem = 0;  // layout of the current store (words per key record,
+                                                             // coefficient planes, bytes per coefficient word)
     Ctrl* ctrl = nullptr;
     Ctrl* ctrl_host = nullptr;  // pinned
     uint32_t* counts = nullptr;

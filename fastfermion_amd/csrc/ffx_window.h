@@ -111,9 +111,9 @@ __device__ __forceinline__ void member_facts(const uint64_t (&kt)[Alg::NP], int 
 }
 
 // Continues the lookup of kt (index hash h) past its home entry, which was
-// examined and does not hold it, and (kValue) loads the candidate's value
-// together with its key words.  Returns the slot among [0, n), or kNone.
-template <class Alg, class CP, int K, bool kValue>
+// examined and does not hold it, and loads the candidate's value together
+// with its key words.  Returns the slot among [0, n), or kNone.
+template <class Alg, class CP, int K>
 __device__ __forceinline__ uint32_t probe_next(const Store<CP, K>& st, const typename CP::store_t* cin,
                                                const uint64_t (&kt)[Alg::NP], uint64_t h, uint32_t n,
                                                typename CP::val_t (&vt)[K]) {
@@ -131,10 +131,8 @@ __device__ __forceinline__ uint32_t probe_next(const Store<CP, K>& st, const typ
         if (e == kEmpty) return kNone;
         const uint32_t idx = (uint32_t)e;
         if ((uint32_t)(e >> 32) == tag && idx < n) {
-            if constexpr (kValue) {
 #pragma unroll
-                for (int kk = 0; kk < K; ++kk) vt[kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, idx);
-            }
+            for (int kk = 0; kk < K; ++kk) vt[kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, idx);
             uint64_t c[NP];
             key_load<NP>(st.key, idx, c);
             bool eq = true;
@@ -209,7 +207,7 @@ __device__ __forceinline__ void orbit_gather(const Store<CP, K>& st, const typen
                 if ((need_probe >> T) & 1) {
                     uint64_t kt[NP];
                     orbit_key<T>(k, w, kt);
-                    slot[T] = probe_next<Alg, CP, K, true>(st, cin, kt, hmix(lk ^ lp_of<T>(w)), n, v[T]);
+                    slot[T] = probe_next<Alg, CP, K>(st, cin, kt, hmix(lk ^ lp_of<T>(w)), n, v[T]);
                     if (slot[T] != kNone) early |= 1u << T;
                 }
             }
