@@ -45,6 +45,9 @@ def parse():
     ap.add_argument("--pruned", action="store_true",
                     help="time the pruned FP32 step instead (for profiling): rotations with |cos| <= 1e-12 as "
                          "fused Clifford maps, truncation after every rotation, mincoeff 1e-4")
+    ap.add_argument("--majorana", action="store_true",
+                    help="time the Majorana path instead (for profiling): 6x6 Fermi-Hubbard, 20 Trotter steps, "
+                         "maxdegree 8, mincoeff 1e-4, FP64 (no CPU leg, no side measurements)")
     return ap.parse_args()
 
 
@@ -258,6 +261,16 @@ def main():
     ctx = Context(0)
     ctx.set_kernel_timing(a.timing_every)
     step = lambda: ctx.pauli_propagate(gates, W, x, z, c, download=False, want_expect=True, **kw)  # noqa: E731
+    if a.majorana:
+        # 72 modes (144 Majoranas, 3 words), t=1, U=4, dt=0.05, observable n_{0,up},
+        # read out on the Fock state with the up spins of the even sites occupied
+        gates, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20)
+        occ = sum(1 << (2 * s) for s in range(36) if s % 2 == 0)
+        occ_w = [occ & ((1 << 64) - 1), occ >> 64]
+        W, nq = 3, 72
+        a.no_cpu = a.no_extras = True
+        step = lambda: ctx.majorana_propagate(gates, W, fm, fc, maxdegree=8, mincoeff=1e-4,  # noqa: E731
+                                              download=False, occ=occ_w)
     if a.pruned:
         from fastfermion_amd._native import prune_opts
 
@@ -291,7 +304,7 @@ def main():
 
     # algorithmic bytes of the propagation kernel: (key + coefficient bytes) per
     # live term read and per live term written, per rotation
-    Kb = 16 * W
+    Kb = (8 if a.majorana else 16) * W
     Cb = {_abi.FP64: 8, _abi.FP32: 4, _abi.FP32C: 8}[prec]
     alg_bytes = (Kb + Cb) * (updates + outs)
     # average duration of the sampled launches; algorithmic bytes per launch
@@ -303,7 +316,8 @@ def main():
     # --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled against a plain copy kernel
     traffic = None
     tf = os.path.join(ROOT, "profiles", "propagation_kicked_ising_10x10_fp64_traffic.json")
-    if (L, layers, a.batched, a.precision) == (10, 20, 1, "fp64") and os.path.exists(tf):
+    if (L, layers, a.batched, a.precision, a.pruned, a.majorana) == (10, 20, 1, "fp64", False, False) \
+            and os.path.exists(tf):
         with open(tf) as fh:
             traffic = json.load(fh)["bytes_per_launch"]
 
@@ -338,13 +352,17 @@ def main():
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f32" if a.pruned else {"fp64": "f64", "fp32": "f32", "fp32c": "f32+f32 compensated"}[a.precision],
-        "data": "synthetic (seeded circuit, single-Pauli observable)",
+        "data": ("synthetic (Fermi-Hubbard Trotter circuit, number-operator observable)" if a.majorana
+                 else "synthetic (seeded circuit, single-Pauli observable)"),
         "config": {
-            "workload": f"kicked-Ising {L}x{L} lattice, {layers} layers, RZZ(-pi/2)+RX(theta_s in [0.2,0.6]), "
-                        f"observable Z_{center}, maxdegree 8, mincoeff 1e-5, "
-                        f"{'batched (reference default)' if a.batched else 'per-gate'} truncation"
-                        + (" [pruned step: FP32, fused Clifford rotations, per-gate truncation, mincoeff 1e-4]"
-                           if a.pruned else ""),
+            "workload": ("Fermi-Hubbard 6x6 (72 modes, 144 Majoranas), t=1, U=4, dt=0.05, 20 Trotter steps, "
+                         "observable n_{0,up}, maxdegree 8, mincoeff 1e-4 after every rotation, FP64"
+                         if a.majorana else
+                         f"kicked-Ising {L}x{L} lattice, {layers} layers, RZZ(-pi/2)+RX(theta_s in [0.2,0.6]), "
+                         f"observable Z_{center}, maxdegree 8, mincoeff 1e-5, "
+                         f"{'batched (reference default)' if a.batched else 'per-gate'} truncation"
+                         + (" [pruned step: FP32, fused Clifford rotations, per-gate truncation, mincoeff 1e-4]"
+                            if a.pruned else "")),
             "qubits": nq,
             "rotations": int(len(gates)),
             "updates_per_step": updates // a.steps,
@@ -359,8 +377,10 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
-            "kernel": "propagation launches: k_win_async (a window of up to 4 commuting rotations) and "
-                      "k_layer_async (a layer of single-qubit rotations)",
+            "kernel": ("propagation launches: k_win_async (a window of up to 4 commuting rotations) and "
+                       "k_rot_async (one rotation)" if a.majorana else
+                       "propagation launches: k_win_async (a window of up to 4 commuting rotations) and "
+                       "k_layer_async (a layer of single-qubit rotations)"),
             "kernel_ms_per_launch": 1e3 * kernel_avg,
             "alg_bytes_per_launch": bytes_per_launch,
             "launches_per_step": kernel_launches // a.steps,
