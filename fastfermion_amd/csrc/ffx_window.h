@@ -615,9 +615,9 @@ __global__ void __launch_bounds__(kBlockWin) k_win_async(Store<CP, K> st, const 
     constexpr uint32_t kPreBlocks = 128;
     const uint32_t begin = blockIdx.x * blockDim.x, step = gridDim.x * blockDim.x;
     const uint32_t i0 = begin + threadIdx.x;
-    const bool kPre = blockIdx.x < kPreBlocks;
+    const bool preload = blockIdx.x < kPreBlocks;
     SlotLoad<Alg::NP, CP, K> ld0{};
-    if (kPre && i0 < st.cap) {
+    if (preload && i0 < st.cap) {
         key_load<Alg::NP>(st.key, i0, ld0.k);
 #pragma unroll
         for (int kk = 0; kk < K; ++kk) ld0.v[kk] = CP::load(cin + (size_t)kk * CP::NS * st.cap, st.cap, i0);
@@ -633,7 +633,8 @@ __global__ void __launch_bounds__(kBlockWin) k_win_async(Store<CP, K> st, const 
     int dd = 0;
     int sdl[kWin] = {0, 0, 0, 0};
     for (uint32_t b0 = begin; b0 < n; b0 += step)
-        win_tile<Alg, CP, K, true>(st, cin, cout, w, n, b0, sink, &sbase, dd, sdl, &stg, kPre && b0 == begin, ld0);
+        win_tile<Alg, CP, K, true>(st, cin, cout, w, n, b0, sink, &sbase, dd, sdl, &stg, preload && b0 == begin,
+                                   ld0);
     stage_flush<Alg::NP, CP, K>(st, cout, sink, n, stg, &sbase);
     window_report(ep, e, dd, sdl);
 }
