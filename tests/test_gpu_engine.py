@@ -308,3 +308,26 @@ def test_layout_switch_after_large_result():
         assert_same_terms(dev, ref, exact=True)
     finally:
         c2.close()
+
+
+def test_compaction_interval_does_not_change_results(ctx, monkeypatch):
+    """With truncation after every rotation the device-decided compaction check
+    runs after every FFX_COMPACT_EVERY-th truncation point: checking every 4th
+    compacts several times, the default interval rarely; terms, coefficients and
+    the work counters must not depend on it (compaction keeps slot order)."""
+    gates, _ = wl.kicked_ising(6, 5)
+    x, z, c = wl.single_pauli("Z", [21], 36)
+    kw = dict(maxdegree=8, mincoeff=1e-4, batched=False)
+    monkeypatch.setenv("FFX_COMPACT_EVERY", "4")
+    often, e_often = ctx.pauli_propagate(gates, 1, x, z, c, want_expect=True, **kw)
+    p_often = ctx.profile()
+    monkeypatch.delenv("FFX_COMPACT_EVERY")
+    dflt, e_dflt = ctx.pauli_propagate(gates, 1, x, z, c, want_expect=True, **kw)
+    p_dflt = ctx.profile()
+    assert p_often["compactions"] > p_dflt["compactions"]
+    assert_same_terms(often, dflt, exact=True, what="compaction interval")
+    for k in ("updates", "outs", "peak_terms"):
+        assert p_often[k] == p_dflt[k], (k, p_often[k], p_dflt[k])
+    assert abs(e_often - e_dflt) < 1e-12
+    orc = pyffo.pauli_propagate(gates, 1, x, z, c, **kw)
+    assert_same_terms(dflt, orc, exact=True, what="compaction interval vs oracle")
