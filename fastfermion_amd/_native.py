@@ -29,6 +29,16 @@ EXPORTED = ("ffx_ctx_create", "ffx_ctx_destroy", "ffx_last_error", "ffx_pauli_pr
             "ffx_pruner_forward", "ffx_pruner_loss_grad", "ffx_pruner_train", "ffx_batch_autotune",
             "ffx_batch_graph_replays")
 
+# maxdegree value that disables the partner degree filter (FFX_NO_DEGREE_LIMIT)
+NO_DEGREE_LIMIT = 0x7FFFFFFF
+
+
+def _md(maxdegree):
+    """None: no filter; an integer is compared literally, as the reference does
+    (src/pauli/propagate.h:82, src/majorana_gates.h:94): a negative one drops every partner."""
+    return NO_DEGREE_LIMIT if maxdegree is None else int(maxdegree)
+
+
 PRUNER_FEATURES = 64
 PRUNER_NPARAMS = 24961
 
@@ -199,7 +209,7 @@ class Context:
         im = np.ctypeslib.as_array(t.c_im, shape=(n,)).copy()
         return Result(x, z, re + 1j * im)
 
-    def pauli_propagate(self, gates, n_words, x, z, c, maxdegree=-1, mincoeff=0.0, topk=0, max_xweight=-1,
+    def pauli_propagate(self, gates, n_words, x, z, c, maxdegree=None, mincoeff=0.0, topk=0, max_xweight=-1,
                         xtrunc_period=1, batched=True, precision=FP64, download=True, state=None,
                         want_expect=False):
         gates = np.ascontiguousarray(gates, dtype=PAULI_GATE_DTYPE)
@@ -208,7 +218,7 @@ class Context:
         c = np.asarray(c, dtype=np.complex128).reshape(-1)
         re = np.ascontiguousarray(c.real)
         im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
-        opts = Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        opts = Options(_md(maxdegree), topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
         t = Terms()
         ex = np.zeros(2)
         st = None
@@ -229,7 +239,7 @@ class Context:
             return res, complex(ex[0], ex[1])
         return res
 
-    def pauli_propagate_batch(self, instances, n_words, maxdegree=-1, mincoeff=0.0, topk=0, max_xweight=-1,
+    def pauli_propagate_batch(self, instances, n_words, maxdegree=None, mincoeff=0.0, topk=0, max_xweight=-1,
                               xtrunc_period=1, batched=True, precision=FP64, slots=0):
         """instances: iterable of (gates, x, z, c, state) with state a list of
         up to 4 uint64 words (or None for |0...0>).  Returns (expectations
@@ -237,7 +247,7 @@ class Context:
         the device-wide engine)."""
         arr, keep = self._instances(instances, n_words)
         n = len(instances)
-        opts = Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        opts = Options(_md(maxdegree), topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
         ex = np.zeros(2 * max(n, 1))
         nt = np.zeros(max(n, 1), np.int64)
         fb = ctypes.c_int32(0)
@@ -246,12 +256,12 @@ class Context:
         del keep
         return ex[0:2 * n:2] + 1j * ex[1:2 * n:2], nt[:n], int(fb.value)
 
-    def batch_autotune(self, instances, n_words, sample=128, maxdegree=-1, mincoeff=0.0, topk=0, max_xweight=-1,
+    def batch_autotune(self, instances, n_words, sample=128, maxdegree=None, mincoeff=0.0, topk=0, max_xweight=-1,
                        xtrunc_period=1, batched=True, precision=FP64):
         """Autotune the batched launch on the longest `sample` instances; later batches
         with slots=0 use the result.  Returns the chosen configuration as a dict."""
         arr, keep = self._instances(instances, n_words)
-        opts = Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        opts = Options(_md(maxdegree), topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
         t = BatchTuning()
         self._check(self._L.ffx_batch_autotune(self._h, arr, len(instances), n_words, ctypes.byref(opts), sample,
                                                ctypes.byref(t)))
@@ -282,7 +292,7 @@ class Context:
                 arr[i].basis_state[w] = int(v)
         return arr, keep
 
-    def majorana_propagate(self, gates, n_words, m, c, maxdegree=-1, mincoeff=0.0, precision=FP64,
+    def majorana_propagate(self, gates, n_words, m, c, maxdegree=None, mincoeff=0.0, precision=FP64,
                            download=True, occ=None, want_expect=False):
         gates = np.ascontiguousarray(gates, dtype=MAJORANA_GATE_DTYPE)
         m = np.ascontiguousarray(m, dtype=np.uint64).reshape(-1, n_words)
@@ -296,7 +306,7 @@ class Context:
             oc = np.zeros(4, np.uint64)
             oc[: len(occ)] = occ
         rc = self._L.ffx_majorana_propagate(self._h, _p(gates), len(gates), n_words, _p(m), _p(re), _p(im),
-                                            len(c), maxdegree, mincoeff, precision,
+                                            len(c), _md(maxdegree), mincoeff, precision,
                                             ctypes.byref(t) if download else None, _p(oc),
                                             _p(ex) if (want_expect or occ is not None) else None)
         self._check(rc)
@@ -313,9 +323,9 @@ class Context:
     # ---- pruned mode, contribution logs, learned pruner ----
     @staticmethod
     def _opts(maxdegree, topk, max_xweight, xtrunc_period, batched, precision, mincoeff):
-        return Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        return Options(_md(maxdegree), topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
 
-    def pauli_propagate_pruned(self, gates, n_words, x, z, c, po, maxdegree=-1, mincoeff=0.0, topk=0, max_xweight=-1,
+    def pauli_propagate_pruned(self, gates, n_words, x, z, c, po, maxdegree=None, mincoeff=0.0, topk=0, max_xweight=-1,
                                xtrunc_period=1, batched=True, precision=FP64, download=True, state=None,
                                want_expect=False):
         """pauli_propagate with the pruned mode of `po` (a PruneOpts)."""
@@ -348,7 +358,7 @@ class Context:
             return res, complex(ex[0], ex[1])
         return res
 
-    def majorana_propagate_pruned(self, gates, n_words, m, c, po, maxdegree=-1, mincoeff=0.0, precision=FP64,
+    def majorana_propagate_pruned(self, gates, n_words, m, c, po, maxdegree=None, mincoeff=0.0, precision=FP64,
                                   download=True, occ=None, want_expect=False):
         gates = np.ascontiguousarray(gates, dtype=MAJORANA_GATE_DTYPE)
         m = np.ascontiguousarray(m, dtype=np.uint64).reshape(-1, n_words)
@@ -362,7 +372,7 @@ class Context:
             oc = np.zeros(4, np.uint64)
             oc[: len(occ)] = occ
         rc = self._L.ffx_majorana_propagate_pruned(self._h, _p(gates), len(gates), n_words, _p(m), _p(re), _p(im),
-                                                   len(c), maxdegree, mincoeff, precision,
+                                                   len(c), _md(maxdegree), mincoeff, precision,
                                                    ctypes.byref(po) if po is not None else None,
                                                    ctypes.byref(t) if download else None, _p(oc),
                                                    _p(ex) if (want_expect or occ is not None) else None)
@@ -383,7 +393,7 @@ class Context:
         sums = np.zeros(2 * n_points)
         return feats, targ, sums
 
-    def pauli_contrib_log(self, gates, n_words, x, z, c, po, log_every=1, max_rows=1 << 20, maxdegree=-1,
+    def pauli_contrib_log(self, gates, n_words, x, z, c, po, log_every=1, max_rows=1 << 20, maxdegree=None,
                           mincoeff=0.0, max_xweight=-1, xtrunc_period=1, batched=True, state=None):
         """Term-contribution log of one plain FP64 propagation: (features float32[n, 64],
         targets float32[n] = clamp(log10(|v| + 1e-12), -12, 0)/4, sums complex[points]
@@ -408,7 +418,7 @@ class Context:
         n, k = nr.value, npo.value
         return feats[:n], targ[:n], sums[0:2 * k:2] + 1j * sums[1:2 * k:2]
 
-    def majorana_contrib_log(self, gates, n_words, m, c, po, log_every=1, max_rows=1 << 20, maxdegree=-1,
+    def majorana_contrib_log(self, gates, n_words, m, c, po, log_every=1, max_rows=1 << 20, maxdegree=None,
                              mincoeff=0.0, occ=None):
         gates = np.ascontiguousarray(gates, dtype=MAJORANA_GATE_DTYPE)
         m = np.ascontiguousarray(m, dtype=np.uint64).reshape(-1, n_words)
@@ -422,7 +432,7 @@ class Context:
             oc[: len(occ)] = occ
         nr, npo = ctypes.c_int64(0), ctypes.c_int32(0)
         self._check(self._L.ffx_majorana_contrib_log(self._h, _p(gates), len(gates), n_words, _p(m), _p(re), _p(im),
-                                                     len(c), maxdegree, mincoeff, ctypes.byref(po), _p(oc), log_every,
+                                                     len(c), _md(maxdegree), mincoeff, ctypes.byref(po), _p(oc), log_every,
                                                      max_rows, _p(feats), _p(targ), ctypes.byref(nr), _p(sums), npts,
                                                      ctypes.byref(npo)))
         n, k = nr.value, npo.value

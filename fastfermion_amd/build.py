@@ -45,18 +45,34 @@ def hip_sources():
         [os.path.join(INCLUDE, "fastfermion_amd.h")]
 
 
+def compile_units():
+    """(source, object name, extra flags) of every translation unit of libffamd.so:
+    each plain csrc/*.hip file once, and ffx_inst.hip once per key width (1..4
+    words), coefficient arithmetic (FP64, FP32, FP32 compensated) and algebra
+    (Pauli, Majorana); the FP64 units, which also hold the logs, go first."""
+    units = []
+    for f in sorted(os.listdir(CSRC)):
+        if f.endswith(".hip") and f != "ffx_inst.hip":
+            units.append((os.path.join(CSRC, f), f[:-4] + ".o", []))
+    inst = os.path.join(CSRC, "ffx_inst.hip")
+    combos = [(w, p, a) for p in range(3) for w in (4, 3, 2, 1) for a in (0, 1)]
+    return [(inst, f"ffx_inst_w{w}_p{p}_a{a}.o", [f"-DFFX_INST_W={w}", f"-DFFX_INST_P={p}",
+                                                   f"-DFFX_INST_ALG={a}"]) for w, p, a in combos] + units
+
+
 def build_lib(force=True, jobs=None):
-    """Compile every csrc/*.hip translation unit in parallel, then link libffamd.so."""
+    """Compile every translation unit in parallel, then link libffamd.so."""
     out = os.path.join(HERE, "libffamd.so")
     if not force and _newer(out, hip_sources()):
         return out
-    units = [os.path.join(CSRC, f) for f in sorted(os.listdir(CSRC)) if f.endswith(".hip")]
+    units = compile_units()
     objdir = os.path.join(HERE, "_build")
     os.makedirs(objdir, exist_ok=True)
-    objs = [os.path.join(objdir, os.path.basename(u)[:-4] + ".o") for u in units]
-    jobs = jobs or min(len(units), max(1, min(16, os.cpu_count() or 1)))
+    objs = [os.path.join(objdir, o) for _, o, _ in units]
+    jobs = jobs or max(1, min(16, os.cpu_count() or 1))
     with ThreadPoolExecutor(jobs) as ex:
-        list(ex.map(lambda uo: _run([HIPCC, *HIP_FLAGS, "-c", "-o", uo[1], uo[0]]), zip(units, objs)))
+        list(ex.map(lambda u: _run([HIPCC, *HIP_FLAGS, *u[2], "-c", "-o", os.path.join(objdir, u[1]), u[0]]),
+                    units))
     tmp = out + ".tmp"
     _run([HIPCC, f"--offload-arch={ARCH}", "-shared", "-o", tmp, *objs])
     os.replace(tmp, out)

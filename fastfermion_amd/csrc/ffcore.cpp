@@ -540,7 +540,7 @@ MajoranaPolynomial propagate_majorana(const std::vector<MRotGate>& circuit, cons
         im[i] = obs.t.vals[i].imag();
         cx = cx || im[i] != 0.0;
     }
-    const int md = maxdegree.has_value() ? maxdegree.value() : -1;
+    const int md = maxdegree.has_value() ? maxdegree.value() : FFX_NO_DEGREE_LIMIT;
     const double mc = mincoeff.has_value() ? mincoeff.value() : 0.0;
     const int prec = precision_code(precision);
     ffx_terms out{};

@@ -182,6 +182,12 @@ inline void free_store(ffx_ctx* ctx) {
     ctx->cap = ctx->hslots = 0;
 }
 
+// Partner degree bound of the kernels: the caller's maxdegree compared literally
+// (src/pauli/propagate.h:82, src/majorana_propagation.h:43; a negative bound drops
+// every partner); FFX_NO_DEGREE_LIMIT and other values above any key's degree
+// become 2^30, which no degree reaches.
+inline int degree_limit(int32_t maxdegree) { return maxdegree > (1 << 30) ? (1 << 30) : maxdegree; }
+
 inline unsigned grid_for(size_t n, size_t per_block, unsigned max_grid = 65535) {
     size_t g = (n + per_block - 1) / per_block;
     if (g < 1) g = 1;
@@ -1489,7 +1495,7 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
     CHK(eng.upload(x, z, re, im, n_terms));
     const double t1 = now_s();
     const std::vector<Step> steps = pauli_schedule<W>(gates, n_gates, o.batched != 0);
-    const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
+    const int maxdeg = degree_limit(o.maxdegree);
     if (!step_engine()) {
         using AStep = typename Engine<Alg, CP, K>::AStep;
         std::vector<AStep> sd;
@@ -1503,7 +1509,7 @@ int run_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const 
         }
         // layers need every term's degree <= kLayerMaxA: no Clifford blocks (they can raise degree),
         // partners capped by maxdegree, observable terms within the bound
-        bool layers = cl.empty() && o.maxdegree >= 0 && o.maxdegree <= kLayerMaxA;
+        bool layers = cl.empty() && o.maxdegree <= kLayerMaxA;
         for (int64_t i = 0; i < n_terms && layers; ++i) {
             int d = 0;
             for (int w = 0; w < W; ++w) d += __builtin_popcountll(x[i * W + w] | z[i * W + w]);
@@ -1564,7 +1570,7 @@ int run_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, 
     const double t0 = now_s();
     CHK(eng.upload(m, nullptr, re, im, n_terms));
     const double t1 = now_s();
-    const int maxdeg = maxdegree < 0 ? (1 << 30) : maxdegree;
+    const int maxdeg = degree_limit(maxdegree);
     if (!step_engine()) {
         using AStep = typename Engine<Alg, CP, K>::AStep;
         std::vector<AStep> sd;
@@ -1650,7 +1656,7 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     using store_t = typename CP::store_t;
     constexpr int NP = Alg::NP;
     const double t0 = now_s();
-    const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
+    const int maxdeg = degree_limit(o.maxdegree);
     // configuration: explicit > autotuned for this (W, precision, K) > defaults
     const int prec_id = std::is_same<CP, F64>::value ? FFX_FP64 : (std::is_same<CP, F32>::value ? FFX_FP32 : FFX_FP32C);
     BatchConf cf{kBlockBatch, kWin, 32768};
@@ -1893,8 +1899,8 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     return FFX_OK;
 }
 
-// Precision and coefficient-width dispatch for one key width W.  Each W is
-// instantiated in its own translation unit.
+// Precision and coefficient-width dispatch for one key width W (instantiated in
+// ffx_lib.hip; the drivers they call are instantiated by ffx_inst.hip).
 template <int W>
 int dispatch_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, const uint64_t* x, const uint64_t* z,
                    const double* re, const double* im, int64_t n_terms, const ffx_options& o, ffx_terms* out,
@@ -2110,7 +2116,7 @@ int dispatch_log_pauli(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gate
     const std::vector<Step> steps = pauli_schedule<W>(gates, n_gates, o.batched != 0);
     for (const Step& st : steps)
         if (st.kind != 0) return set_err(ctx, FFX_EINVAL, "contribution logs need a circuit of rotations only");
-    const int maxdeg = o.maxdegree < 0 ? (1 << 30) : o.maxdegree;
+    const int maxdeg = degree_limit(o.maxdegree);
     std::vector<CliffGate> cl;
     if (K == 1) {
         std::vector<typename Engine<PauliAlg<W>, F64, 1>::AStep> sd;
@@ -2143,7 +2149,7 @@ int dispatch_log_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t 
                           const ffx_prune_opts& po, const uint64_t* occ, int log_every, int64_t max_rows,
                           float* feats, float* targets, int64_t* n_rows, double* sums, int max_points,
                           int32_t* n_points, int K) {
-    const int maxdeg = maxdegree < 0 ? (1 << 30) : maxdegree;
+    const int maxdeg = degree_limit(maxdegree);
     if (K == 1) {
         std::vector<typename Engine<MajoranaAlg<W>, F64, 1>::AStep> sd;
         majorana_log_steps<W, 1>(gates, n_gates, mincoeff, sd);
@@ -2182,23 +2188,53 @@ int dispatch_expect(ffx_ctx* ctx, const uint64_t* state, double* out, int kind, 
     return K == 1 ? expect_last_impl<M, F32C, 1>(ctx, state, out) : expect_last_impl<M, F32C, 2>(ctx, state, out);
 }
 
-#define FFX_DISPATCH_DECL(EXT, W_)                                                                                  \
-    EXT template int dispatch_pauli<W_>(ffx_ctx*, const ffx_pauli_gate*, int64_t, const uint64_t*, const uint64_t*, \
-                                        const double*, const double*, int64_t, const ffx_options&, ffx_terms*,     \
-                                        const uint64_t*, double*, int, int, const ffx_prune_opts*);                \
-    EXT template int dispatch_majorana<W_>(ffx_ctx*, const ffx_majorana_gate*, int64_t, const uint64_t*,           \
-                                           const double*, const double*, int64_t, int, double, ffx_terms*,         \
-                                           const uint64_t*, double*, int, int, const ffx_prune_opts*);             \
-    EXT template int dispatch_log_pauli<W_>(ffx_ctx*, const ffx_pauli_gate*, int64_t, const uint64_t*,            \
-                                            const uint64_t*, const double*, const double*, int64_t,                \
-                                            const ffx_options&, const ffx_prune_opts&, const uint64_t*, int,       \
-                                            int64_t, float*, float*, int64_t*, double*, int, int32_t*, int);       \
-    EXT template int dispatch_log_majorana<W_>(ffx_ctx*, const ffx_majorana_gate*, int64_t, const uint64_t*,       \
-                                               const double*, const double*, int64_t, int, double,                 \
-                                               const ffx_prune_opts&, const uint64_t*, int, int64_t, float*,       \
-                                               float*, int64_t*, double*, int, int32_t*, int);                     \
-    EXT template int dispatch_expect<W_>(ffx_ctx*, const uint64_t*, double*, int, int, int);                      \
-    EXT template int dispatch_batch<W_>(ffx_ctx*, const ffx_instance*, int32_t, const ffx_options&, int64_t,       \
-                                        double*, int64_t*, int32_t*, int, int, const BatchConf*)
+// Explicit instantiations of the drivers: for one key width W_ and one coefficient
+// arithmetic CP_ (both coefficient kinds) of each algebra, and the FP64
+// contribution logs of one algebra.  EXT = extern declares them.  ffx_inst.hip is
+// compiled once per (key width, arithmetic, algebra), so the kernels compile in
+// parallel; ffx_lib.hip declares them all extern and instantiates the dispatchers.
+#define FFX_PAULI_INST(EXT, W_, CP_)                                                                                \
+    EXT template int run_pauli<W_, CP_, 1>(ffx_ctx*, const ffx_pauli_gate*, int64_t, const uint64_t*, const uint64_t*, \
+                                           const double*, const double*, int64_t, const ffx_options&, ffx_terms*,     \
+                                           const uint64_t*, double*, const ffx_prune_opts*);                          \
+    EXT template int run_pauli<W_, CP_, 2>(ffx_ctx*, const ffx_pauli_gate*, int64_t, const uint64_t*, const uint64_t*, \
+                                           const double*, const double*, int64_t, const ffx_options&, ffx_terms*,     \
+                                           const uint64_t*, double*, const ffx_prune_opts*);                          \
+    EXT template int run_pauli_batch<W_, CP_, 1>(ffx_ctx*, const ffx_instance*, int32_t, const ffx_options&, int64_t, \
+                                                 double*, int64_t*, int32_t*, const BatchConf*);                      \
+    EXT template int run_pauli_batch<W_, CP_, 2>(ffx_ctx*, const ffx_instance*, int32_t, const ffx_options&, int64_t, \
+                                                 double*, int64_t*, int32_t*, const BatchConf*);                      \
+    EXT template int expect_last_impl<PauliAlg<W_>, CP_, 1>(ffx_ctx*, const uint64_t*, double*);                      \
+    EXT template int expect_last_impl<PauliAlg<W_>, CP_, 2>(ffx_ctx*, const uint64_t*, double*)
+
+#define FFX_MAJORANA_INST(EXT, W_, CP_)                                                                             \
+    EXT template int run_majorana<W_, CP_, 1>(ffx_ctx*, const ffx_majorana_gate*, int64_t, const uint64_t*,          \
+                                              const double*, const double*, int64_t, int, double, ffx_terms*,        \
+                                              const uint64_t*, double*, const ffx_prune_opts*);                      \
+    EXT template int run_majorana<W_, CP_, 2>(ffx_ctx*, const ffx_majorana_gate*, int64_t, const uint64_t*,          \
+                                              const double*, const double*, int64_t, int, double, ffx_terms*,        \
+                                              const uint64_t*, double*, const ffx_prune_opts*);                      \
+    EXT template int expect_last_impl<MajoranaAlg<W_>, CP_, 1>(ffx_ctx*, const uint64_t*, double*);                   \
+    EXT template int expect_last_impl<MajoranaAlg<W_>, CP_, 2>(ffx_ctx*, const uint64_t*, double*)
+
+#define FFX_LOG_INST(EXT, ALG_)                                                                                     \
+    EXT template int run_contrib_log<ALG_, 1>(ffx_ctx*, std::vector<typename Engine<ALG_, F64, 1>::AStep>&,          \
+                                              const uint64_t*, const uint64_t*, const double*, const double*, int64_t, \
+                                              double, int, int, const ffx_prune_opts&, const uint64_t*, int, int64_t,  \
+                                              float*, float*, int64_t*, double*, int, int32_t*);                      \
+    EXT template int run_contrib_log<ALG_, 2>(ffx_ctx*, std::vector<typename Engine<ALG_, F64, 2>::AStep>&,          \
+                                              const uint64_t*, const uint64_t*, const double*, const double*, int64_t, \
+                                              double, int, int, const ffx_prune_opts&, const uint64_t*, int, int64_t,  \
+                                              float*, float*, int64_t*, double*, int, int32_t*)
+
+#define FFX_EXTERN_WIDTH(W_)                                                                                         \
+    FFX_PAULI_INST(extern, W_, F64);                                                                                 \
+    FFX_PAULI_INST(extern, W_, F32);                                                                                 \
+    FFX_PAULI_INST(extern, W_, F32C);                                                                                \
+    FFX_MAJORANA_INST(extern, W_, F64);                                                                              \
+    FFX_MAJORANA_INST(extern, W_, F32);                                                                              \
+    FFX_MAJORANA_INST(extern, W_, F32C);                                                                             \
+    FFX_LOG_INST(extern, PauliAlg<W_>);                                                                              \
+    FFX_LOG_INST(extern, MajoranaAlg<W_>)
 
 }  // namespace ffx_host

@@ -1,15 +1,15 @@
 // ffx_lib.hip -- C ABI of libffamd.so (declared in include/fastfermion_amd.h):
 // argument checks, then dispatch on the key width to the drivers instantiated
-// in ffx_w1.hip .. ffx_w4.hip.
+// by ffx_inst.hip (one object per key width, arithmetic and algebra).
 #include "ffx_engine.h"
 
 using namespace ffx_host;
 
 namespace ffx_host {
-FFX_DISPATCH_DECL(extern, 1);
-FFX_DISPATCH_DECL(extern, 2);
-FFX_DISPATCH_DECL(extern, 3);
-FFX_DISPATCH_DECL(extern, 4);
+FFX_EXTERN_WIDTH(1);
+FFX_EXTERN_WIDTH(2);
+FFX_EXTERN_WIDTH(3);
+FFX_EXTERN_WIDTH(4);
 }  // namespace ffx_host
 
 // ---------------------------------------------------------------------------

@@ -39,6 +39,10 @@ extern "C" {
 
 #define FFX_MAX_WORDS 4
 
+/* maxdegree value that disables the partner degree filter (the reference's
+ * maxdegree=None of the Majorana overload, src/python_bind.cpp:505-511). */
+#define FFX_NO_DEGREE_LIMIT 0x7FFFFFFF
+
 enum ffx_status {
     FFX_OK = 0,
     FFX_EINVAL = -1,    /* invalid argument (shape, index, duplicate keys, ...) */
@@ -82,7 +86,9 @@ typedef struct ffx_majorana_gate {
 /* Truncation and execution options; the first five fields are the reference's
  * propagate keyword arguments (src/pauli/propagate.h:297-300). */
 typedef struct ffx_options {
-    int32_t maxdegree;      /* partner degree filter (propagate.h:82); < 0: none */
+    int32_t maxdegree;      /* partner degree filter (propagate.h:82): partners of degree > maxdegree
+                               are dropped, compared literally (a negative value drops every
+                               partner); FFX_NO_DEGREE_LIMIT: no filter */
     int32_t topk;           /* keep the topk largest |c| at each truncation; 0: off */
     int32_t max_xweight;    /* X/Y-weight cap (truncate.h:41-46); -1: off */
     int32_t xtrunc_period;  /* the cap applies when (rot_count+1) % period == 0 */
@@ -140,7 +146,7 @@ int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gat
 
 /* Majorana propagation.  Replaces majorana_gates::propagate
  * (src/majorana_propagation.h:18-55) behind ffcore.propagate overload #2
- * (src/python_bind.cpp:490-527).  maxdegree < 0 means no degree filter; the
+ * (src/python_bind.cpp:490-527).  maxdegree as in ffx_options; the
  * threshold applies after every gate when mincoeff > 0.  When out_expect is
  * non-NULL, <occ|O|occ> for the Fock state with mode occupations occ (NULL: vacuum)
  * is computed on the device (src/fockstate.h:95-114). */

@@ -29,6 +29,8 @@ extern "C" {
 enum { FFO_ROT = 0, FFO_H = 1, FFO_S = 2, FFO_CNOT = 3, FFO_SWAP = 4, FFO_CZ = 5 };
 enum { FFO_SERIAL = 0, FFO_SHARDED = 1, FFO_OMP = 2 };
 
+#define FFO_NO_DEGREE_LIMIT 0x7FFFFFFF
+
 typedef struct ffo_pauli_gate {
     int32_t kind;      /* FFO_ROT or one of the Clifford kinds */
     int32_t q0, q1;    /* Clifford qubits (q1 unused for H/S) */
@@ -46,7 +48,8 @@ typedef struct ffo_majorana_gate {
 } ffo_majorana_gate;
 
 typedef struct ffo_options {
-    int32_t maxdegree;      /* partner degree filter; < 0 means no filter */
+    int32_t maxdegree;      /* partner degree filter, compared literally (a negative value drops
+                               every partner); FFO_NO_DEGREE_LIMIT: no filter */
     int32_t topk;           /* keep the topk largest |c| per truncation (0 = off) */
     int32_t max_xweight;    /* X/Y-weight cap (-1 = off) */
     int32_t xtrunc_period;  /* apply the X/Y-weight cap every this many rotations */
@@ -76,7 +79,7 @@ int ffo_pauli_propagate(const ffo_pauli_gate* gates, int64_t n_gates, int32_t n_
                         const double* c_im, int64_t n_terms, const ffo_options* opts,
                         int32_t strategy, int32_t n_threads, ffo_terms* out);
 
-/* Majorana propagation (src/majorana_propagation.h:18-55). */
+/* Majorana propagation (src/majorana_propagation.h:18-55); maxdegree as in ffo_options. */
 int ffo_majorana_propagate(const ffo_majorana_gate* gates, int64_t n_gates, int32_t n_words,
                            const uint64_t* m, const double* c_re, const double* c_im,
                            int64_t n_terms, int32_t maxdegree, double mincoeff, ffo_terms* out);
@@ -89,8 +92,9 @@ void ffo_majorana_expect(const ffo_terms* t, const uint64_t* occ, double out[2])
 
 void ffo_free(ffo_terms* t);
 
-/* Live terms entering each rotation of the last propagate call, in application
- * order (serial strategy); returns the rotation count, writes at most cap. */
+/* Live terms entering each rotation of the calling thread's last propagate call,
+ * in application order (serial strategy); returns the rotation count, writes at
+ * most cap.  Propagate calls from different threads may run concurrently. */
 int64_t ffo_rotation_counts(int64_t* out, int64_t cap);
 
 /* Threads the OpenMP runtime would use by default (0 when built without OpenMP). */

@@ -36,8 +36,8 @@ namespace {
 
 using cplx = std::complex<double>;
 
-// Work counters of one propagate call (not thread-safe across concurrent calls;
-// the oracle is single-call test infrastructure).
+// Work counters of one propagate call, one set per calling thread: concurrent
+// calls from several host threads (one circuit each) keep separate counters.
 struct Stats {
     int64_t updates = 0;
     int64_t peak = 0;
@@ -48,7 +48,7 @@ struct Stats {
         n_in_per_rotation.push_back(n_in);
     }
 };
-Stats g_stats;
+thread_local Stats g_stats;
 
 inline int popc(uint64_t v) { return __builtin_popcountll(v); }
 
@@ -564,7 +564,8 @@ PMap<W> pauli_propagate(const ffo_pauli_gate* circuit, int64_t n, PMap<W> obs, c
     n_threads = 1;
 #endif
     if (n_threads <= 1) strategy = FFO_SERIAL;
-    const int maxdegree = o.maxdegree < 0 ? 1 << 30 : o.maxdegree;
+    // compared literally, as at src/pauli/propagate.h:82 (FFO_NO_DEGREE_LIMIT = INT32_MAX disables it)
+    const int maxdegree = o.maxdegree;
     int64_t clifford_begin = 0;
     bool pending = false;
     int rot_count = 0;
@@ -715,7 +716,7 @@ MMap<W> majorana_propagate(const ffo_majorana_gate* circuit, int64_t n, MMap<W> 
                 MKey<W> prod;
                 for (int w = 0; w < W; ++w) prod.m[w] = ms.m[w] ^ x.m[w];
                 cplx mxcoef(m1pow(merge_parity<W>(x, ms)), 0);
-                if (maxdegree < 0 || mdegree<W>(prod) <= maxdegree)
+                if (mdegree<W>(prod) <= maxdegree)  // filter_pred, src/majorana_propagation.h:43
                     o_new.emplace_back(prod, ret.vals[t] * iirsintheta * mxcoef);
                 ret.vals[t] *= costheta;
             }

@@ -24,6 +24,14 @@ assert PAULI_GATE_DTYPE.itemsize == 88 and MAJORANA_GATE_DTYPE.itemsize == 48
 
 SERIAL, SHARDED, OMP = 0, 1, 2
 
+# maxdegree value that disables the partner degree filter
+NO_DEGREE_LIMIT = 0x7FFFFFFF
+
+
+def _md(maxdegree):
+    """None: no filter; an integer is compared literally (src/pauli/propagate.h:82)."""
+    return NO_DEGREE_LIMIT if maxdegree is None else int(maxdegree)
+
 _HERE = os.path.dirname(os.path.abspath(__file__))
 _LIB = None
 
@@ -44,11 +52,12 @@ class _Terms(ctypes.Structure):
                 ("peak_terms", ctypes.c_int64)]
 
 
-def lib():
-    """Load (building first if needed) the oracle library."""
+def lib(path=None):
+    """Load (building first if needed) the oracle library.  `path` (first call
+    only) loads another build of the same source, e.g. one with a planted mistake."""
     global _LIB
     if _LIB is None:
-        path = os.path.join(_HERE, "libffo_oracle.so")
+        path = path or os.path.join(_HERE, "libffo_oracle.so")
         if not os.path.exists(path):
             import subprocess
             subprocess.run(["make", "-s", "-C", _HERE], check=True)
@@ -105,7 +114,7 @@ def _collect(t: _Terms, pauli: bool) -> Result:
 
 
 def pauli_propagate(gates: np.ndarray, n_words: int, x: np.ndarray, z: np.ndarray, c: np.ndarray,
-                    maxdegree: int = -1, mincoeff: float = 0.0, topk: int = 0, max_xweight: int = -1,
+                    maxdegree: int | None = None, mincoeff: float = 0.0, topk: int = 0, max_xweight: int = -1,
                     xtrunc_period: int = 1, batched: bool = True, strategy: int = SERIAL,
                     n_threads: int = 1) -> Result:
     L = lib()
@@ -115,7 +124,7 @@ def pauli_propagate(gates: np.ndarray, n_words: int, x: np.ndarray, z: np.ndarra
     c = np.asarray(c, dtype=np.complex128).reshape(-1)
     re = np.ascontiguousarray(c.real)
     im = np.ascontiguousarray(c.imag)
-    opts = _Options(maxdegree, topk, max_xweight, xtrunc_period, int(bool(batched)), 0, mincoeff)
+    opts = _Options(_md(maxdegree), topk, max_xweight, xtrunc_period, int(bool(batched)), 0, mincoeff)
     t = _Terms()
     rc = L.ffo_pauli_propagate(_ptr(gates), len(gates), n_words, _ptr(x), _ptr(z), _ptr(re), _ptr(im),
                                len(c), ctypes.byref(opts), strategy, n_threads, ctypes.byref(t))
@@ -128,7 +137,7 @@ def pauli_propagate(gates: np.ndarray, n_words: int, x: np.ndarray, z: np.ndarra
 
 
 def majorana_propagate(gates: np.ndarray, n_words: int, m: np.ndarray, c: np.ndarray,
-                       maxdegree: int = -1, mincoeff: float = 0.0) -> Result:
+                       maxdegree: int | None = None, mincoeff: float = 0.0) -> Result:
     L = lib()
     gates = np.ascontiguousarray(gates, dtype=MAJORANA_GATE_DTYPE)
     m = np.ascontiguousarray(m, dtype=np.uint64).reshape(-1, n_words)
@@ -137,7 +146,7 @@ def majorana_propagate(gates: np.ndarray, n_words: int, m: np.ndarray, c: np.nda
     im = np.ascontiguousarray(c.imag)
     t = _Terms()
     rc = L.ffo_majorana_propagate(_ptr(gates), len(gates), n_words, _ptr(m), _ptr(re), _ptr(im), len(c),
-                                  maxdegree, mincoeff, ctypes.byref(t))
+                                  _md(maxdegree), mincoeff, ctypes.byref(t))
     if rc != 0:
         raise RuntimeError(f"oracle majorana propagate failed ({rc})")
     try:

@@ -219,3 +219,19 @@ def test_profile_dictionary_has_reference_keys():
     assert d["n_gates"] == len(circ)
     assert d["updates"] > 0 and d["emit"] >= 0.0
     ff.profile_disable()
+
+
+def test_negative_maxdegree_is_literal():
+    """maxdegree compares literally in both overloads (src/pauli/propagate.h:82,
+    src/majorana_propagation.h:43, bound at src/python_bind.cpp:404, :505-511):
+    -1 keeps no partner, so one rotation leaves only the cos-scaled terms."""
+    import math
+    t = 0.37
+    res = propagate([ROT("X", [0], t)], ff.poly("Z0 + 0.5 Z1"), maxdegree=-1)
+    assert len(res) == 2
+    assert abs(res.coefficient(ff.PauliString("ZI")) - math.cos(t)) < 1e-15
+    assert len(propagate([ROT("X", [0], t)], ff.poly("Z0 + 0.5 Z1"))) == 3
+    circ = [MROT(MajoranaString([0, 1]), 1j, t)]
+    mres = propagate(circ, ff.poly("m1 m2"), maxdegree=-1)
+    assert len(mres) == 1
+    assert len(propagate(circ, ff.poly("m1 m2"))) == 2

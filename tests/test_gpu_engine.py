@@ -73,6 +73,9 @@ CASES = [
     ("tfim6", lambda: tfim(6, 0.1), 5, [("Z", [0], 1.0)], 6, {}),
     ("tfim6_thresh", lambda: tfim(6, 0.1), 5, [("Z", [0], 1.0)], 6, {"mincoeff": 1e-3}),
     ("tfim6_maxdeg", lambda: tfim(6, 0.1), 5, [("Z", [0], 1.0)], 6, {"maxdegree": 3}),
+    # maxdegree is compared literally (src/pauli/propagate.h:82): no partner survives -1 or 0
+    ("tfim6_maxdeg_neg", lambda: tfim(6, 0.1), 5, [("Z", [0], 1.0), ("ZZ", [1, 2], 0.5)], 6, {"maxdegree": -1}),
+    ("tfim6_maxdeg0", lambda: tfim(6, 0.1), 5, [("Z", [0], 1.0)], 6, {"maxdegree": 0}),
     ("heis4", lambda: heisenberg(4, 0.03), 5, [("Z", [0], 1.0)], 4, {}),
     ("poly_obs", lambda: tfim(4, 0.05), 10, [("Z", [0], 1.0), ("Z", [1], 1.0)], 4, {}),
     ("identity_rot", lambda: rot("X", [0], 0.0), 10, [("Z", [0], 1.0)], 1, {}),
@@ -331,3 +334,46 @@ def test_compaction_interval_does_not_change_results(ctx, monkeypatch):
     assert abs(e_often - e_dflt) < 1e-12
     orc = pyffo.pauli_propagate(gates, 1, x, z, c, **kw)
     assert_same_terms(dflt, orc, exact=True, what="compaction interval vs oracle")
+
+
+@pytest.mark.parametrize("maxdegree", [-1, 1])
+def test_kicked_ising_literal_maxdegree(ctx, maxdegree):
+    """Window and layer kernels with a bound below every term's degree: only the
+    cos-scaled copies survive, identical to the oracle."""
+    g, _ = wl.kicked_ising(L=4, layers=3)
+    x, z, c = wl.single_pauli("ZZ", [5, 6], 16)
+    for batched in (True, False):
+        ref, dev = both(ctx, g, 1, x, z, c, maxdegree=maxdegree, batched=batched)
+        assert_same_terms(dev, ref, exact=True, what=f"maxdegree={maxdegree} batched={batched}")
+    assert len(ref) == 1  # every partner of Z5 Z6 has degree >= 2
+
+
+def test_majorana_literal_maxdegree(ctx):
+    rng = np.random.default_rng(11)
+    g = random_majorana(rng, 12, 30)
+    m, c = mobs([([0, 1], 1.0), ([2, 5], 0.5j)], 12)
+    for md in (-1, 0, 2):
+        ref = pyffo.majorana_propagate(g, 1, m, c, maxdegree=md)
+        dev = ctx.majorana_propagate(g, 1, m, c, maxdegree=md)
+        assert_same_terms(dev, ref, exact=True, what=f"maxdegree={md}")
+
+
+@pytest.mark.parametrize("batched", [True, False])
+def test_complex_threshold_boundary(ctx, batched):
+    """|c| <= mincoeff for a complex coefficient (src/pauli/truncate.h:16-18) at
+    the boundary: c = (3 + 4i) 2^-20 has |c| = 5 * 2^-20 exactly.  mincoeff equal
+    to it erases the term, one ulp below keeps it, in the one-rotation kernel and
+    inside a window of commuting rotations.  (Magnitudes that are not exactly
+    representable can round differently in glibc's hypot and the device's; the
+    reference's std::abs is glibc's hypot, which is not correctly rounded.)"""
+    e = 2.0 ** -20
+    x, z, c = obs([("Z", [0], (3 + 4j) * e), ("Z", [1], 0.5 + 0.5j), ("X", [2], 1.0)], 4)
+    # a truncation point that leaves the boundary term untouched: Z rotations commute with Z0
+    single = rot("Z", [3], 0.3)
+    window = np.concatenate([rot("Z", [3], 0.3), rot("ZZ", [1, 3], 0.2), rot("Z", [2], 0.4), rot("ZZ", [0, 1], 0.1)])
+    for g in (single, window):
+        for mc, kept in ((5 * e, False), (math.nextafter(5 * e, 0.0), True)):
+            ref, dev = both(ctx, g, 1, x, z, c, mincoeff=mc, batched=batched)
+            assert_same_terms(dev, ref, exact=True, what=f"mincoeff={mc!r}")
+            has = any(int(dev.x[i, 0]) == 0 and int(dev.z[i, 0]) == 1 for i in range(len(dev)))
+            assert has == kept, (len(g), mc)

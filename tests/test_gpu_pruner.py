@@ -148,7 +148,7 @@ def test_log_influence_exact_without_truncation(ctx):
     x, z, c = wl.single_pauli("ZX", [1, 3], n)
     row, col = [0] * n, list(range(n))
     po = prune_opts(row, col, 1, rot_per_layer=8)
-    feats, targ, sums = ctx.pauli_contrib_log(gates, 1, x, z, c, po, log_every=5, maxdegree=-1, mincoeff=0.0,
+    feats, targ, sums = ctx.pauli_contrib_log(gates, 1, x, z, c, po, log_every=5, maxdegree=None, mincoeff=0.0,
                                               batched=False)
     # dense: <0|U^dag O U|0> with U the circuit (first gate applied first)
     U = np.eye(2 ** n, dtype=complex)

@@ -189,14 +189,107 @@ def test_xweight_truncation_period():
     assert len(free) > len(res)
 
 
-def test_topk_keeps_largest():
-    n = 4
-    tf = [("ZZ", (i, (i + 1) % n), 0.3) for i in range(n)] + [("X", (i,), 0.4) for i in range(n)]
-    g = np.concatenate([wl._pack(tf)] * 2)
-    spec = {D.words_from_letters("ZIII"): 1.0}
-    full = run1(g, spec, batched=True)
-    res = run1(g, spec, batched=False, topk=10)
-    assert len(res) <= 10
+def pauli_key(letters):
+    return D.words_from_letters(letters)
+
+
+def test_maxdegree_negative_drops_every_partner():
+    """maxdegree is compared literally (src/pauli/propagate.h:82): -1 keeps only the
+    cos-scaled terms; NO_DEGREE_LIMIT keeps the partner."""
+    t = 0.37
+    res = as_dict(run1(rot("XI", t), {pauli_key("ZI"): 1.0, pauli_key("IZ"): 0.5}, maxdegree=-1))
+    assert res == {pauli_key("ZI"): math.cos(t), pauli_key("IZ"): 0.5}
+    res = as_dict(run1(rot("XI", t), {pauli_key("ZI"): 1.0}, maxdegree=pyffo.NO_DEGREE_LIMIT))
+    assert res == {pauli_key("ZI"): math.cos(t), pauli_key("YI"): math.sin(t)}
+    # Majorana: filter_pred(a) = a.degree() <= maxdegree (src/majorana_propagation.h:43)
+    mres, _ = mrun(mrot([0, 1], t), {0b0110: 1.0}, maxdegree=-1)
+    assert list(mres) == [0b0110] and mres[0b0110] == math.cos(t)
+
+
+def test_topk_keeps_exactly_the_k_largest():
+    """truncate_top_k (src/pauli/truncate.h:21-38) on distinct magnitudes: after
+    RX(t) on qubit 0, O = Z0 + a1 Z1 + a2 Z2 + a3 Z3 holds cos t Z0, sin t Y0 and
+    the untouched terms, so the kept set is known in closed form.  A selection of
+    the smallest terms, or one that keeps k+1 terms, fails."""
+    t = 0.3
+    a = {"ZIII": 1.0, "IZII": 0.5, "IIZI": 0.25, "IIIZ": 0.125}
+    spec = {pauli_key(k): v for k, v in a.items()}
+    allv = {"ZIII": math.cos(t), "YIII": math.sin(t), "IZII": 0.5, "IIZI": 0.25, "IIIZ": 0.125}
+    order = sorted(allv, key=lambda k: -abs(allv[k]))
+    for k in range(1, 6):
+        for batched in (True, False):
+            res = as_dict(run1(rot("XIII", t), spec, topk=k, batched=batched))
+            assert res == {pauli_key(q): allv[q] for q in order[:k]}, (k, batched)
+
+
+def test_topk_runs_at_every_truncation_point():
+    """Per-gate schedule: top-k after each rotation (propagate.h:331-333), so a term
+    that ranks k+1-th after the first rotation is gone even if it would rank in
+    the final top k; one commuting batch truncates once (propagate.h:343)."""
+    # application order (reverse walk): RX on qubit 0 (0.4), then RX on qubit 1 (0.7)
+    g = np.concatenate([rot("IX", 0.7), rot("XI", 0.4)])
+    spec = {pauli_key("ZII"): 1.0, pauli_key("IZI"): 0.5, pauli_key("IIZ"): 0.1}
+    c1, s1, c2, s2 = math.cos(0.4), math.sin(0.4), math.cos(0.7), math.sin(0.7)
+    # per gate: after RX0 keep {Z0 c1, Z1 0.5} (Y0 s1 = 0.389 is third); after RX1:
+    # {Z0 c1, Z1 0.5 c2 = 0.382, Y1 0.5 s2 = 0.322} -> keep {Z0, Z1}
+    res = as_dict(run1(g, spec, topk=2, batched=False))
+    assert res == {pauli_key("ZII"): c1, pauli_key("IZI"): 0.5 * c2}
+    # one batch: {Z0 c1, Y0 s1 = 0.389, Z1 0.382, Y1 0.322, Z2 0.1} -> keep {Z0, Y0}
+    res = as_dict(run1(g, spec, topk=2, batched=True))
+    assert res == {pauli_key("ZII"): c1, pauli_key("YII"): s1}
+
+
+def test_batch_schedule_closed_form():
+    """Commuting batches of the REVERSED gate list, truncated once per batch
+    (propagate.h:272-291, 337-346), against per-gate truncation (:331-333).
+
+    Forward list [A = Z1, B = X0(b), C = X1, D = X0(d)]; the walk applies D, C, B, A.
+    Greedy batches of the reversed list: {D, C, B}, {A}.  Greedy batches of the
+    forward list would be {A, B}, {C, D}.  With |sin d| <= mincoeff < |sin(d+b)|:
+      reversed batches: D's small Y0 partner survives to B, which rotates it back
+        (B and D share the axis X0): Z0 -> cos(d+b) Z0 + sin(d+b) Y0;
+      per gate, or forward batches: Y0 is erased after D (or after {D, C}):
+        Z0 -> cos d cos b Z0 + cos d sin b Y0.
+    The two differ by sin d sin b = 6.4e-4."""
+    d, b = 1e-3, 0.7
+    g = np.concatenate([rot("IZ", 0.2), rot("XI", b), rot("IX", 0.3), rot("XI", d)])
+    spec = {pauli_key("ZI"): 1.0}
+    res = as_dict(run1(g, spec, mincoeff=2e-3, batched=True))
+    assert set(res) == {pauli_key("ZI"), pauli_key("YI")}
+    assert abs(res[pauli_key("ZI")] - math.cos(d + b)) < 1e-15
+    assert abs(res[pauli_key("YI")] - math.sin(d + b)) < 1e-15
+    res = as_dict(run1(g, spec, mincoeff=2e-3, batched=False))
+    assert set(res) == {pauli_key("ZI"), pauli_key("YI")}
+    assert abs(res[pauli_key("ZI")] - math.cos(d) * math.cos(b)) < 1e-15
+    assert abs(res[pauli_key("YI")] - math.cos(d) * math.sin(b)) < 1e-15
+    # without a threshold both schedules are exact: the rotation by d + b
+    for batched in (True, False):
+        res = as_dict(run1(g, spec, batched=batched))
+        assert abs(res[pauli_key("ZI")] - math.cos(d + b)) < 1e-15
+        assert abs(res[pauli_key("YI")] - math.sin(d + b)) < 1e-15
+
+
+def test_xweight_cap_uses_last_rotation_of_a_batch():
+    """truncate_all gets rot_count - 1 after a batch, i.e. the index of the batch's
+    last rotation (propagate.h:343); the cap applies when (index + 1) % period == 0
+    (truncate.h:55).  Walk order: X0(a), Z1 | Y0(c) -> batches {0, 1}, {2}."""
+    a, c = 0.5, 0.3
+    g = np.concatenate([rot("YI", c), rot("IZ", 0.2), rot("XI", a)])
+    spec = {pauli_key("ZI"): 1.0}
+    ca, sa, cc, sc = math.cos(a), math.sin(a), math.cos(c), math.sin(c)
+    # period 2: cap after the first batch (last index 1) erases Y0; the second batch
+    # (index 2) is not capped, so X0 from Y(c) stays
+    res = as_dict(run1(g, spec, max_xweight=0, xtrunc_period=2, batched=True))
+    assert set(res) == {pauli_key("ZI"), pauli_key("XI")}
+    assert abs(res[pauli_key("ZI")] - ca * cc) < 1e-15
+    assert abs(abs(res[pauli_key("XI")]) - ca * sc) < 1e-15
+    # period 3: only the second batch is capped
+    res = as_dict(run1(g, spec, max_xweight=0, xtrunc_period=3, batched=True))
+    assert res == {pauli_key("ZI"): ca * cc}
+    # period 4: never
+    res = as_dict(run1(g, spec, max_xweight=0, xtrunc_period=4, batched=True))
+    assert set(res) == {pauli_key("ZI"), pauli_key("XI"), pauli_key("YI")}
+    assert abs(res[pauli_key("YI")] - sa) < 1e-15
 
 
 def tfim(n, dt):
@@ -325,3 +418,26 @@ def test_majorana_truncation_semantics():
     # |c| == mincoeff is erased after the gate
     res, _ = mrun(mrot([0, 1], 0.4), {0b1110: 1.0}, mincoeff=abs(math.sin(0.4)))
     assert list(res) == [0b1110]
+
+
+def test_majorana_threshold_after_every_gate_closed_form():
+    """The Majorana driver erases |c| <= mincoeff after EVERY gate
+    (src/majorana_propagation.h:24-31).  Two rotations about the same axis m0 m1
+    (angles a then b, |sin a| <= mincoeff): per-gate truncation erases the small
+    partner after the first, leaving cos a * (rotation by b); truncation once at
+    the end would give the rotation by a + b."""
+    a, b, modes = 1e-3, 0.6, 2
+    g = wl.pack_majorana([([0, 1], b, 1), ([0, 1], a, 1)])  # the walk applies a, then b
+    T = [1, 2]
+    O = D.majorana_matrix(T, modes)
+    per_b = D.majorana_decompose(mrot_unitary([0, 1], b, modes).conj().T @ O @ mrot_unitary([0, 1], b, modes), modes)
+    whole = D.majorana_decompose(mrot_unitary([0, 1], a + b, modes).conj().T @ O @ mrot_unitary([0, 1], a + b, modes),
+                                 modes)
+    res, _ = mrun(g, {0b0110: 1.0}, mincoeff=2e-3)
+    assert set(res) == set(per_b) == {0b0110, 0b0101}
+    for k in res:
+        assert abs(res[k] - math.cos(a) * per_b[k]) < 1e-15, k
+        assert abs(res[k] - whole[k]) > 1e-4, k
+    res, _ = mrun(g, {0b0110: 1.0})
+    for k in set(res) | set(whole):
+        assert abs(res.get(k, 0) - whole.get(k, 0)) < 1e-15, k
