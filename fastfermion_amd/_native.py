@@ -27,7 +27,7 @@ EXPORTED = ("ffx_ctx_create", "ffx_ctx_destroy", "ffx_last_error", "ffx_pauli_pr
             "ffx_pauli_propagate_pruned", "ffx_majorana_propagate_pruned", "ffx_pauli_contrib_log",
             "ffx_majorana_contrib_log", "ffx_pruner_init", "ffx_pruner_set_params", "ffx_pruner_get_params",
             "ffx_pruner_forward", "ffx_pruner_loss_grad", "ffx_pruner_train", "ffx_batch_autotune",
-            "ffx_batch_graph_replays")
+            "ffx_batch_graph_replays", "ffx_pauli_propagate_noalloc")
 
 # maxdegree value that disables the partner degree filter (FFX_NO_DEGREE_LIMIT)
 NO_DEGREE_LIMIT = 0x7FFFFFFF
@@ -138,6 +138,8 @@ def load():
             L.ffx_batch_autotune.argtypes = [vp, ctypes.POINTER(Instance), i32, i32, ctypes.POINTER(Options), i32,
                                              ctypes.POINTER(BatchTuning)]
             L.ffx_batch_graph_replays.argtypes = [vp, ctypes.POINTER(ctypes.c_int64)]
+            L.ffx_pauli_propagate_noalloc.argtypes = [vp, vp, i64, i32, vp, vp, vp, vp, i64, ctypes.POINTER(Options),
+                                                      vp, vp, vp, vp, vp, vp, i64, ctypes.POINTER(ctypes.c_int64)]
             L.ffx_pruner_init.argtypes = [vp, ctypes.c_uint64]
             L.ffx_pruner_set_params.argtypes = [vp, vp, i64]
             L.ffx_pruner_get_params.argtypes = [vp, vp, i64]
@@ -238,6 +240,39 @@ class Context:
         if want_expect or state is not None:
             return res, complex(ex[0], ex[1])
         return res
+
+    def pauli_propagate_noalloc(self, gates, n_words, x, z, c, maxdegree=None, mincoeff=0.0, topk=0, max_xweight=-1,
+                                xtrunc_period=1, batched=True, precision=FP64, state=None, out_cap=None):
+        """ffx_pauli_propagate_noalloc: no device allocation; the terms come back in
+        arrays of out_cap terms allocated here (None: 1 << 16; 0: expectation and
+        term count only).  Returns (Result or None, <b|O|b>, live term count)."""
+        gates = np.ascontiguousarray(gates, dtype=PAULI_GATE_DTYPE)
+        x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, n_words)
+        z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, n_words)
+        c = np.asarray(c, dtype=np.complex128).reshape(-1)
+        re = np.ascontiguousarray(c.real)
+        im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+        opts = Options(_md(maxdegree), topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
+        cap = 1 << 16 if out_cap is None else int(out_cap)
+        ox = np.zeros((cap, n_words), np.uint64)
+        oz = np.zeros((cap, n_words), np.uint64)
+        ore = np.zeros(cap)
+        oim = np.zeros(cap)
+        ex = np.zeros(2)
+        st = np.zeros(4, np.uint64)
+        if state is not None:
+            st[: len(state)] = state
+        n = ctypes.c_int64(0)
+        rc = self._L.ffx_pauli_propagate_noalloc(self._h, _p(gates), len(gates), n_words, _p(x), _p(z), _p(re), _p(im),
+                                                 len(c), ctypes.byref(opts), _p(st), _p(ex),
+                                                 _p(ox) if cap else None, _p(oz) if cap else None,
+                                                 _p(ore) if cap else None, _p(oim) if cap else None, cap,
+                                                 ctypes.byref(n))
+        self.last_noalloc_terms = int(n.value)
+        self._check(rc)
+        k = int(n.value)
+        res = Result(ox[:k].copy(), oz[:k].copy(), ore[:k] + 1j * oim[:k]) if cap else None
+        return res, complex(ex[0], ex[1]), k
 
     def pauli_propagate_batch(self, instances, n_words, maxdegree=None, mincoeff=0.0, topk=0, max_xweight=-1,
                               xtrunc_period=1, batched=True, precision=FP64, slots=0):

@@ -117,6 +117,21 @@ struct ffx_ctx {
     uint32_t layer_stamp = 0;   // stamp of the last layer launch (24 bits)
     void* layer_buf = nullptr;  // layer records, qubit-position tables, per-rotation counters
     size_t layer_buf_bytes = 0;
+    // allocation-free calls (ffx_pauli_propagate_noalloc): while `on`, device blocks are
+    // carved from [base, base + size) of the device's fixed scratch array instead of
+    // hipMalloc, and the term store has exactly `cap` slots
+    struct Arena {
+        char* base = nullptr;
+        size_t size = 0, used = 0, cap = 0;
+        bool on = false;
+    } arena;
+    // ... and the propagated terms go to caller-owned host arrays of `cap` terms
+    struct Sink {
+        uint64_t *x = nullptr, *z = nullptr;
+        double *re = nullptr, *im = nullptr;
+        int64_t cap = 0, n = 0;
+        bool on = false;
+    } sink;
 };
 
 // Releases the pruner model of a context (defined with the pruner kernels).
@@ -154,10 +169,35 @@ inline size_t pow2ceil(size_t v) {
     return p;
 }
 
+// Device ranges of the fixed scratch arrays of allocation-free calls, one per
+// device, registered when a device's array is first used; dev_free never
+// releases memory inside them.
+constexpr int kMaxDevices = 64;
+inline std::atomic<uintptr_t> g_fixed_lo[kMaxDevices] = {};
+inline std::atomic<uintptr_t> g_fixed_hi[kMaxDevices] = {};
+
+inline bool in_fixed_scratch(const void* p) {
+    const uintptr_t a = (uintptr_t)p;
+    for (int d = 0; d < kMaxDevices; ++d)
+        if (a >= g_fixed_lo[d].load(std::memory_order_relaxed) && a < g_fixed_hi[d].load(std::memory_order_relaxed))
+            return true;
+    return false;
+}
+
 template <class T>
 int dev_alloc(ffx_ctx* ctx, T** p, size_t bytes) {
     *p = nullptr;
     if (bytes == 0) bytes = 256;
+    if (ctx->arena.on) {
+        const size_t off = (ctx->arena.used + 255) & ~(size_t)255;
+        if (off > ctx->arena.size || bytes > ctx->arena.size - off)
+            return set_err(ctx, FFX_ECAPACITY,
+                           "allocation-free call: its scratch of " + std::to_string(ctx->arena.size) +
+                               " bytes cannot hold another block of " + std::to_string(bytes) + " bytes");
+        *p = (T*)(ctx->arena.base + off);
+        ctx->arena.used = off + bytes;
+        return FFX_OK;
+    }
     hipError_t e = hipMalloc((void**)p, bytes);
     if (e != hipSuccess) {
         *p = nullptr;
@@ -169,7 +209,7 @@ int dev_alloc(ffx_ctx* ctx, T** p, size_t bytes) {
 
 template <class T>
 void dev_free(T*& p) {
-    if (p) (void)hipFree((void*)p);
+    if (p && !in_fixed_scratch((const void*)p)) (void)hipFree((void*)p);
     p = nullptr;
 }
 
@@ -187,6 +227,63 @@ inline void free_store(ffx_ctx* ctx) {
 // every partner); FFX_NO_DEGREE_LIMIT and other values above any key's degree
 // become 2^30, which no degree reaches.
 inline int degree_limit(int32_t maxdegree) { return maxdegree > (1 << 30) ? (1 << 30) : maxdegree; }
+
+// The device buffers a context allocates on demand (term store, counters,
+// staging, schedule tables).  An allocation-free call swaps an empty set in,
+// fills it from the scratch array, and swaps the context's own set back.
+struct DevBuffers {
+    uint64_t *key = nullptr, *key2 = nullptr, *index = nullptr;
+    void *coef = nullptr, *coef2 = nullptr;
+    size_t cap = 0, hslots = 0, key_planes = 0, coef_planes = 0, coef_elem = 0;
+    Ctrl* ctrl = nullptr;
+    uint32_t *counts = nullptr, *total = nullptr, *epochs = nullptr, *topk_buf = nullptr;
+    size_t counts_cap = 0, epochs_cap = 0;
+    double *partial = nullptr, *result = nullptr;
+    void* staging = nullptr;
+    size_t staging_bytes = 0;
+    CliffGate* cliff = nullptr;
+    size_t cliff_cap = 0;
+    void* crot = nullptr;
+    size_t crot_bytes = 0;
+    uint64_t* claim = nullptr;
+    size_t claim_slots = 0;
+    uint32_t layer_stamp = 0;
+    void* layer_buf = nullptr;
+    size_t layer_buf_bytes = 0;
+};
+
+inline void swap_buffers(ffx_ctx* c, DevBuffers& b) {
+    std::swap(c->key, b.key);
+    std::swap(c->key2, b.key2);
+    std::swap(c->index, b.index);
+    std::swap(c->coef, b.coef);
+    std::swap(c->coef2, b.coef2);
+    std::swap(c->cap, b.cap);
+    std::swap(c->hslots, b.hslots);
+    std::swap(c->key_planes, b.key_planes);
+    std::swap(c->coef_planes, b.coef_planes);
+    std::swap(c->coef_elem, b.coef_elem);
+    std::swap(c->ctrl, b.ctrl);
+    std::swap(c->counts, b.counts);
+    std::swap(c->total, b.total);
+    std::swap(c->epochs, b.epochs);
+    std::swap(c->topk_buf, b.topk_buf);
+    std::swap(c->counts_cap, b.counts_cap);
+    std::swap(c->epochs_cap, b.epochs_cap);
+    std::swap(c->partial, b.partial);
+    std::swap(c->result, b.result);
+    std::swap(c->staging, b.staging);
+    std::swap(c->staging_bytes, b.staging_bytes);
+    std::swap(c->cliff, b.cliff);
+    std::swap(c->cliff_cap, b.cliff_cap);
+    std::swap(c->crot, b.crot);
+    std::swap(c->crot_bytes, b.crot_bytes);
+    std::swap(c->claim, b.claim);
+    std::swap(c->claim_slots, b.claim_slots);
+    std::swap(c->layer_stamp, b.layer_stamp);
+    std::swap(c->layer_buf, b.layer_buf);
+    std::swap(c->layer_buf_bytes, b.layer_buf_bytes);
+}
 
 inline unsigned grid_for(size_t n, size_t per_block, unsigned max_grid = 65535) {
     size_t g = (n + per_block - 1) / per_block;
@@ -413,6 +510,13 @@ struct Engine {
         // (key width, precision, real/complex) keeps the capacity the context already had
         if (same_layout && ncap < 2 * ctx->cap) ncap = std::max<size_t>(ncap, 2 * ctx->cap);
         if (!same_layout) ncap = std::max<size_t>(ncap, ctx->cap);
+        if (ctx->arena.on) {  // one store of the planned size; no growth
+            if (need > ctx->arena.cap || ctx->cap != 0)
+                return set_err(ctx, FFX_ECAPACITY,
+                               "allocation-free call: the term store needs more than the " +
+                                   std::to_string(ctx->arena.cap) + " slots its scratch holds");
+            ncap = ctx->arena.cap;
+        }
         uint64_t *nkey = nullptr, *nkey2 = nullptr, *nindex = nullptr;
         void *ncoef = nullptr, *ncoef2 = nullptr;
         const size_t cbytes = coef_planes() * ncap * sizeof(store_t);
@@ -610,6 +714,7 @@ struct Engine {
         }
         const size_t nt = total;
         const size_t kb = nt * W * sizeof(uint64_t);
+        if (ctx->sink.on) return download_to_sink(nt, nb);
         out->x = (uint64_t*)malloc(std::max<size_t>(kb, 8));
         out->z = Alg::kPauli ? (uint64_t*)malloc(std::max<size_t>(kb, 8)) : nullptr;
         out->c_re = (double*)malloc(std::max<size_t>(nt * sizeof(double), 8));
@@ -634,6 +739,35 @@ struct Engine {
         if (Alg::kPauli) HIPCHK(hipMemcpyAsync(out->z, oz, kb, hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipMemcpyAsync(out->c_re, ore, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipMemcpyAsync(out->c_im, oim, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        return FFX_OK;
+    }
+
+    // Live terms [0, nt) packed into caller-owned host arrays (allocation-free calls);
+    // counts holds the per-tile live counts of the download's scan.
+    int download_to_sink(size_t nt, size_t nb) {
+        constexpr int W = Alg::W;
+        ffx_ctx::Sink& sk = ctx->sink;
+        sk.n = (int64_t)nt;
+        if ((int64_t)nt > sk.cap)
+            return set_err(ctx, FFX_ECAPACITY,
+                           "the result has " + std::to_string(nt) + " terms; the output arrays hold " +
+                               std::to_string(sk.cap));
+        if (nt == 0) return FFX_OK;
+        const size_t kb = nt * W * sizeof(uint64_t);
+        CHK(ensure_staging((Alg::kPauli ? 2 : 1) * kb + 2 * nt * sizeof(double)));
+        char* sp = (char*)ctx->staging;
+        uint64_t* ox = (uint64_t*)sp;
+        uint64_t* oz = Alg::kPauli ? (uint64_t*)(sp + kb) : nullptr;
+        double* ore = (double*)(sp + (Alg::kPauli ? 2 : 1) * kb);
+        double* oim = ore + nt;
+        hipLaunchKernelGGL((k_pack<Alg, CP, K>), dim3(nb), dim3(kBlock), 0, ctx->stream, store(), ctx->counts, ox, oz,
+                           ore, oim);
+        CHK(launch_check());
+        if (sk.x) HIPCHK(hipMemcpyAsync(sk.x, ox, kb, hipMemcpyDeviceToHost, ctx->stream));
+        if (Alg::kPauli && sk.z) HIPCHK(hipMemcpyAsync(sk.z, oz, kb, hipMemcpyDeviceToHost, ctx->stream));
+        if (sk.re) HIPCHK(hipMemcpyAsync(sk.re, ore, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
+        if (sk.im) HIPCHK(hipMemcpyAsync(sk.im, oim, nt * sizeof(double), hipMemcpyDeviceToHost, ctx->stream));
         HIPCHK(hipStreamSynchronize(ctx->stream));
         return FFX_OK;
     }
@@ -2226,6 +2360,12 @@ int dispatch_expect(ffx_ctx* ctx, const uint64_t* state, double* out, int kind, 
                                               const uint64_t*, const uint64_t*, const double*, const double*, int64_t, \
                                               double, int, int, const ffx_prune_opts&, const uint64_t*, int, int64_t,  \
                                               float*, float*, int64_t*, double*, int, int32_t*)
+
+// Argument checks of the Pauli entry points (ffx_lib.hip); makes the context's
+// device current and sets *K_out = 2 when any imaginary part is nonzero.
+int check_pauli_args(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words, const uint64_t* x,
+                     const uint64_t* z, const double* c_re, const double* c_im, int64_t n_terms,
+                     const ffx_options* opts, int* K_out);
 
 #define FFX_EXTERN_WIDTH(W_)                                                                                         \
     FFX_PAULI_INST(extern, W_, F64);                                                                                 \

@@ -223,6 +223,14 @@ int majorana_propagate_impl(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_
 
 }  // namespace
 
+namespace ffx_host {
+int check_pauli_args(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words, const uint64_t* x,
+                     const uint64_t* z, const double* c_re, const double* c_im, int64_t n_terms,
+                     const ffx_options* opts, int* K_out) {
+    return check_pauli(ctx, gates, n_gates, n_words, x, z, c_re, c_im, n_terms, opts, K_out);
+}
+}  // namespace ffx_host
+
 extern "C" {
 
 int ffx_pauli_propagate(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,

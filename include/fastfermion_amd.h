@@ -231,6 +231,34 @@ int ffx_set_kernel_timing(ffx_ctx* ctx, int every);
 /* Number of visible HIP devices (0 when none); never fails. */
 int ffx_device_count(void);
 
+/* Bytes of the fixed scratch array of ffx_pauli_propagate_noalloc, held on each
+ * device from the first such call there until the process ends. */
+#define FFX_NOALLOC_SCRATCH_BYTES (128ull * 1024 * 1024)
+
+/* ffx_pauli_propagate without device memory allocation.  No counterpart in the
+ * reference; it is for callers that cannot allocate device memory: code that runs
+ * under HIP graph capture, tight serving loops, harnesses that forbid hipMalloc.
+ * Same schedule, truncation, kernels and results as ffx_pauli_propagate (no pruned
+ * mode).  Every device block of the call -- term store, hash index, counters,
+ * epoch arrays, schedule tables, staging -- is carved from a fixed scratch array of
+ * FFX_NOALLOC_SCRATCH_BYTES per device; the call calls no hipMalloc or hipFree (its
+ * host-side schedule uses the C++ heap).  A problem whose term store would not fit
+ * (planned from the key width, precision, gate and term counts: e.g. 524288 slots
+ * for 128-bit keys in FP64 with 5600 gates) fails with FFX_ECAPACITY before any
+ * launch, or when the live terms outgrow the planned store.  The propagated terms
+ * go to caller-owned host arrays: out_x, out_z (out_cap * n_words words each),
+ * out_re and optionally out_im (out_cap doubles); with out_cap = 0 (arrays may be
+ * NULL) only out_expect / out_n are produced.  *out_n receives the live term count;
+ * a result of more than out_cap terms fails with FFX_ECAPACITY after setting it.
+ * All calls on one GPU share its scratch array and leave their contents in it:
+ * calls that overlap on one GPU (several threads or contexts) race.  The call
+ * leaves no device-resident result for ffx_expect_last. */
+int ffx_pauli_propagate_noalloc(ffx_ctx* ctx, const ffx_pauli_gate* gates, int64_t n_gates, int32_t n_words,
+                                const uint64_t* x, const uint64_t* z, const double* c_re, const double* c_im,
+                                int64_t n_terms, const ffx_options* opts, const uint64_t* basis_state,
+                                double out_expect[2], uint64_t* out_x, uint64_t* out_z, double* out_re,
+                                double* out_im, int64_t out_cap, int64_t* out_n);
+
 /* ------------------------------------------------------------------------
  * Pruned propagation and the learned term pruner (no reference counterpart:
  * the reference truncates by degree, |c|, top-k and X/Y weight only,
