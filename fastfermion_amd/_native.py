@@ -291,10 +291,11 @@ class Context:
         del keep
         return ex[0:2 * n:2] + 1j * ex[1:2 * n:2], nt[:n], int(fb.value)
 
-    def batch_autotune(self, instances, n_words, sample=128, maxdegree=None, mincoeff=0.0, topk=0, max_xweight=-1,
+    def batch_autotune(self, instances, n_words, sample=0, maxdegree=None, mincoeff=0.0, topk=0, max_xweight=-1,
                        xtrunc_period=1, batched=True, precision=FP64):
-        """Autotune the batched launch on the longest `sample` instances; later batches
-        with slots=0 use the result.  Returns the chosen configuration as a dict."""
+        """Autotune the batched launch on the whole batch (sample = 0) or on the
+        `sample` longest instances; later batches with slots=0 use the result.
+        Returns the chosen configuration as a dict."""
         arr, keep = self._instances(instances, n_words)
         opts = Options(_md(maxdegree), topk, max_xweight, xtrunc_period, int(bool(batched)), precision, mincoeff)
         t = BatchTuning()

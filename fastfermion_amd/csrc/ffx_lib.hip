@@ -378,12 +378,12 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     if (n_inst <= 0 || !out) return set_err(ctx, FFX_EINVAL, "invalid arguments");
     int K = 1;
     CHK(check_batch(ctx, inst, n_inst, n_words, opts, &K));
-    // sample: the instances with the longest circuits (they bound a batch's time)
+    // timed batch: all instances (sample <= 0), or the sample with the longest circuits
     std::vector<int32_t> order((size_t)n_inst);
     for (int32_t i = 0; i < n_inst; ++i) order[(size_t)i] = i;
     std::stable_sort(order.begin(), order.end(),
                      [&](int32_t a, int32_t b) { return inst[a].n_gates > inst[b].n_gates; });
-    const int32_t m = std::min<int32_t>(n_inst, sample > 0 ? sample : 128);
+    const int32_t m = sample > 0 ? std::min<int32_t>(n_inst, sample) : n_inst;
     std::vector<ffx_instance> sub((size_t)m);
     for (int32_t i = 0; i < m; ++i) sub[(size_t)i] = inst[order[(size_t)i]];
     std::vector<double> ex(2 * (size_t)m);

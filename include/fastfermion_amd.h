@@ -171,7 +171,8 @@ typedef struct ffx_instance {
  * circuit per propagate call, src/pauli/propagate.h:297-493, and a batch is a
  * loop over such calls).  Every instance runs the reference schedule and
  * truncation of ffx_pauli_propagate with the shared options; all instances
- * run in one launch, one workgroup each, with `slots` term slots per instance
+ * run in one launch, one workgroup each (largest estimated work first:
+ * rotations^2 / qubits), with `slots` term slots per instance
  * (0: a default).  Instances whose terms outgrow that budget, that contain
  * Clifford gates or that ask for top-k are run afterwards on the device-wide
  * engine, so every result is complete.  The upload, the launch and the download
@@ -194,11 +195,12 @@ typedef struct ffx_batch_tuning {
 } ffx_batch_tuning;
 
 /* Autotuner of batched propagation (no reference counterpart): times every
- * combination of workgroup size, window length and slot budget on the `sample`
- * instances with the longest circuits (sample <= 0: 128) with the options of
- * `opts`, prefers configurations without device-wide fallbacks, and keeps the
- * fastest on the context for this key width, precision and coefficient kind;
- * later ffx_pauli_propagate_batch calls with slots = 0 use it. */
+ * combination of workgroup size, window length and slot budget on the whole
+ * batch (sample <= 0) or on the `sample` instances with the longest circuits,
+ * with the options of `opts`; prefers configurations without device-wide
+ * fallbacks, and keeps the fastest on the context for this key width, precision
+ * and coefficient kind; later ffx_pauli_propagate_batch calls with slots = 0
+ * use it. */
 int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
                        const ffx_options* opts, int32_t sample, ffx_batch_tuning* out);
 
