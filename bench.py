@@ -47,6 +47,9 @@ def parse():
     ap.add_argument("--pruned", action="store_true",
                     help="time the pruned FP32 step instead (for profiling): rotations with |cos| <= 1e-12 as "
                          "fused Clifford maps, truncation after every rotation, mincoeff 1e-4")
+    ap.add_argument("--learned", action="store_true",
+                    help="run only the learned-pruner measurements (logs, device training, pruned runs) and print "
+                         "them as one JSON object (for profiling the MLP kernels)")
     ap.add_argument("--majorana", action="store_true",
                     help="time the Majorana path instead (for profiling): 6x6 Fermi-Hubbard, 20 Trotter steps, "
                          "maxdegree 8, mincoeff 1e-4, FP64 (no CPU leg, no side measurements)")
@@ -254,18 +257,19 @@ def side_measurements(ctx, gates, W, x, z, c, plain_ms, plain_expect, cpu_legs=T
     dt64, (e64, _, _) = _time(lambda: ctx.pauli_propagate_batch(items, Wb, **kw), reps=2)
     dt_default, _ = _time(lambda: ctx.pauli_propagate_batch(items, Wb, precision=_abi.FP32C, **kw), reps=2)
     ta = time.perf_counter()
-    tuned = ctx.batch_autotune(items, Wb, sample=64, precision=_abi.FP32C, **kw)
+    tuned = ctx.batch_autotune(items, Wb, sample=0, precision=_abi.FP32C, **kw)
     tune_s = time.perf_counter() - ta
     r0 = ctx.batch_graph_replays()
     dt32, (e32, _, fb) = _time(lambda: ctx.pauli_propagate_batch(items, Wb, precision=_abi.FP32C, **kw), reps=3)
     upd = ctx.profile()["updates"]
+    graph_ms = 1e3 * ctx.profile()["seconds_kernel"]
     replays = ctx.batch_graph_replays() - r0
     d = np.abs(e32 - e64)
     out["batch"] = {"instances": len(items), "workload": "kicked-Ising L in 6..12, depth 4..24, RX angles in "
                     "[0, pi/2], weight-1/2 Z observable, maxdegree 8, mincoeff 1e-4, per-rotation truncation, "
                     "one workgroup per instance, one launch", "precision": "fp32c", "ms": 1e3 * dt32,
                     "updates": int(upd), "updates_per_s": upd / dt32, "fallback_instances": fb,
-                    "fp64_ms": 1e3 * dt64, "max_abs_err_vs_fp64": float(d.max()),
+                    "graph_ms": graph_ms, "fp64_ms": 1e3 * dt64, "max_abs_err_vs_fp64": float(d.max()),
                     "median_abs_err_vs_fp64": float(np.median(d)), "default_launch_ms": 1e3 * dt_default,
                     "autotuned": dict(tuned, seconds=tune_s), "graph_replays_of_3_timed": replays}
     return out
@@ -340,6 +344,21 @@ def pruned_measurements(ctx, gates, W, x, z, c, plain_ms, plain_expect):
     return out
 
 
+def pareto(plain, learned):
+    """Does the learned pruner beat |c| thresholds at equal error?  For every
+    learned run, the fastest plain run with an error no larger; gain = its time /
+    the learned run's time (> 1: the network wins at that error)."""
+    rows = []
+    for r in learned:
+        ok = [p for p in plain if p["abs_err"] <= r["abs_err"]]
+        best = min(ok, key=lambda p: p["ms"]) if ok else None
+        rows.append({"log10_tau": r["log10_tau"], "abs_err": r["abs_err"], "ms": r["ms"],
+                     "best_plain_ms_at_no_larger_err": best["ms"] if best else None,
+                     "gain": (best["ms"] / r["ms"]) if best else None})
+    gains = [q["gain"] for q in rows if q["gain"] is not None]
+    return {"equal_error_comparison": rows, "learned_beats_plain": bool(gains) and max(gains) > 1.0}
+
+
 def learned_measurements(ctx, gates, W, x, z, c, plain_expect):
     """Learned pruner: contribution logs of kicked-Ising circuits of the workload's
     family (theta seeded in [0.2, 0.6] and [0.3, 0.7]), device training, ranking
@@ -353,17 +372,17 @@ def learned_measurements(ctx, gates, W, x, z, c, plain_expect):
     row, col = _GEO["rowcol"]
     obs, rpl = _GEO["obs"], _GEO["rpl"]
     L = _GEO["L"]
-    out = {"model": "MLP 64-128-128-1, ReLU, bf16 weights, fp32 accumulate (MFMA 16x16x16 bf16)"}
+    out = {"model": "MLP 64-128-128-1, ReLU, bf16 weights, fp32 accumulate (MFMA 16x16x32 bf16)"}
     po = prune_opts(row, col, obs, rot_per_layer=rpl)
     trunc = dict(maxdegree=8, mincoeff=1e-5, batched=False)
     circ = [wl.kicked_ising(L, 20 if s % 2 == 0 else 30, theta_lo=0.2 + 0.1 * (s % 2), theta_hi=0.6 + 0.1 * (s % 2),
                             seed=100 + s)[0] for s in range(8)]
     t = time.perf_counter()
-    F, T = pruner.pauli_logs(ctx, circ[:7], W, x, z, c, po, log_every=20, **trunc)
+    F, T = pruner.pauli_logs(ctx, circ[:7], W, x, z, c, po, log_every=5, **trunc)
     Fh, Th = pruner.pauli_logs(ctx, circ[7:], W, x, z, c, po, log_every=20, **trunc)
     out["logs"] = {"rows": int(len(F)), "held_out_rows": int(len(Fh)), "seconds": time.perf_counter() - t,
                    "circuits": "7 kicked-Ising 10x10 (20 layers theta in [0.2,0.6] / 30 layers theta in [0.3,0.7]), "
-                               "per-gate truncation, every 20th rotation logged; held out: an 8th circuit"}
+                               "per-gate truncation, every 5th rotation logged; held out: an 8th circuit, every 20th"}
     ctx.pruner_init(3)
     t = time.perf_counter()
     hist = ctx.pruner_train(F, T, epochs=5, batch=4096, lr=1e-3, seed=1, under_weight=32.0)
@@ -388,23 +407,28 @@ def learned_measurements(ctx, gates, W, x, z, c, plain_expect):
     out["held_out_removed_contribution_share"] = {
         str(f): dict(zip(("by_abs_coefficient", "by_learned_score"), pruner.removed_share(Fh, Th, y, f)))
         for f in (0.5, 0.8, 0.9)}
-    # the 30-layer theta in [0.3, 0.7] circuit
+    # Pareto study on the 30-layer theta in [0.3, 0.7] circuit: plain |c| thresholds
+    # against the learned pruner (every layer), both with the Clifford-angle rotations
+    # fused and FP32 compensated coefficients, errors against the FP64 reference run
     g5, _ = wl.kicked_ising_family(L)
     kw = dict(maxdegree=8, download=False, want_expect=True)
     d0, (_, e0) = _time(lambda: ctx.pauli_propagate(g5, W, x, z, c, mincoeff=1e-5, batched=True, **kw), 2)
-    runs = {"plain_fp64_batched": {"ms": 1e3 * d0, "expectation": e0.real, "updates": int(ctx.profile()["updates"])}}
-    for mc in (1e-5, 3e-4, 1e-3):
+    ref = {"ms": 1e3 * d0, "expectation": e0.real, "updates": int(ctx.profile()["updates"]),
+           "what": "plain FP64, reference schedule, mincoeff 1e-5 (bit-identical to the oracle)"}
+    plain, learned = [], []
+    for mc in (1e-6, 1e-5, 3e-5, 1e-4, 3e-4, 1e-3):
         pc = prune_opts(row, col, obs, clifford_tol=1e-12, rot_per_layer=rpl)
         d1, (_, e1) = _time(lambda: ctx.pauli_propagate_pruned(g5, W, x, z, c, pc, mincoeff=mc, batched=False,
                                                                precision=_abi.FP32C, **kw))
-        runs[f"clifford_fp32c_mincoeff_{mc:g}"] = {"ms": 1e3 * d1, "abs_err": abs(e1 - e0),
-                                                  "updates": int(ctx.profile()["updates"])}
-    for tau in (-6.0, -5.0, -4.0):
+        plain.append({"mincoeff": mc, "ms": 1e3 * d1, "abs_err": abs(e1 - e0), "updates": int(ctx.profile()["updates"])})
+    for tau in (-7.0, -6.0, -5.0, -4.0):
         pl = prune_opts(row, col, obs, clifford_tol=1e-12, rot_per_layer=rpl, prune_period=rpl, log10_tau=tau)
         d2, (_, e2) = _time(lambda: ctx.pauli_propagate_pruned(g5, W, x, z, c, pl, mincoeff=1e-5, batched=False,
                                                                precision=_abi.FP32C, **kw))
-        runs[f"learned_tau_{tau:g}"] = {"ms": 1e3 * d2, "abs_err": abs(e2 - e0), "updates": int(ctx.profile()["updates"])}
-    out["kicked_ising_10x10_30_layers"] = runs
+        learned.append({"log10_tau": tau, "mincoeff": 1e-5, "ms": 1e3 * d2, "abs_err": abs(e2 - e0),
+                        "updates": int(ctx.profile()["updates"])})
+    out["kicked_ising_10x10_30_layers"] = {"reference": ref, "plain": plain, "learned": learned,
+                                           **pareto(plain, learned)}
     # Fermi-Hubbard 6x6: model trained on logs of U = 3 and U = 5, applied at U = 4
     fh, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20)
     frow, fcol = wl.hubbard_geometry(6, 6)
@@ -417,17 +441,23 @@ def learned_measurements(ctx, gates, W, x, z, c, plain_expect):
                                   log_every=116, max_rows=1_500_000, maxdegree=8, mincoeff=1e-4, occ=occw)
     ctx.pruner_init(5)
     fh_hist = ctx.pruner_train(FF, FT, epochs=4, batch=4096, lr=1e-3, seed=2, under_weight=32.0)
-    fruns = {"log_rows": int(len(FF)), "log_and_train_seconds": time.perf_counter() - t,
-             "epoch_loss": [float(h) for h in fh_hist]}
     fk = dict(maxdegree=8, download=False, occ=occw)
     d3, (_, e3) = _time(lambda: ctx.majorana_propagate(fh, 3, fm, fc, mincoeff=1e-4, **fk), 2)
-    fruns["plain_fp64"] = {"ms": 1e3 * d3, "expectation": e3.real, "updates": int(ctx.profile()["updates"])}
-    for tau in (-7.0, -6.0, -5.0):
+    fref = {"ms": 1e3 * d3, "expectation": e3.real, "updates": int(ctx.profile()["updates"]),
+            "what": "plain FP64, mincoeff 1e-4 (bit-identical to the oracle)"}
+    fplain, flearned = [], []
+    for mc in (1e-4, 2e-4, 5e-4, 1e-3, 2e-3):
+        d5, (_, e5) = _time(lambda: ctx.majorana_propagate(fh, 3, fm, fc, mincoeff=mc, precision=_abi.FP32C, **fk), 2)
+        fplain.append({"mincoeff": mc, "ms": 1e3 * d5, "abs_err": abs(e5 - e3), "updates": int(ctx.profile()["updates"])})
+    for tau in (-8.0, -7.0, -6.0, -5.0):
         pf = prune_opts(frow, fcol, 0, rot_per_layer=frpl, prune_period=frpl, log10_tau=tau)
         d4, (_, e4) = _time(lambda: ctx.majorana_propagate_pruned(fh, 3, fm, fc, pf, mincoeff=1e-4, precision=_abi.FP32C,
                                                                   **fk), 2)
-        fruns[f"learned_fp32c_tau_{tau:g}"] = {"ms": 1e3 * d4, "abs_err": abs(e4 - e3),
-                                              "updates": int(ctx.profile()["updates"])}
+        flearned.append({"log10_tau": tau, "mincoeff": 1e-4, "ms": 1e3 * d4, "abs_err": abs(e4 - e3),
+                         "updates": int(ctx.profile()["updates"])})
+    fruns = {"log_rows": int(len(FF)), "log_and_train_seconds": time.perf_counter() - t,
+             "epoch_loss": [float(h) for h in fh_hist], "reference": fref, "plain": fplain, "learned": flearned,
+             **pareto(fplain, flearned)}
     out["fermi_hubbard_6x6"] = fruns
     return out
 
@@ -457,6 +487,10 @@ def main():
     kw = dict(maxdegree=8, mincoeff=1e-5, batched=bool(a.batched), precision=prec)
 
     ctx = Context(0)
+    if a.learned:
+        print(json.dumps({"learned_pruner": learned_measurements(ctx, gates, W, x, z, c, None)}), flush=True)
+        ctx.close()
+        return
     ctx.set_kernel_timing(a.timing_every)
     step = lambda: ctx.pauli_propagate(gates, W, x, z, c, download=False, want_expect=True, **kw)  # noqa: E731
     if a.majorana:

@@ -1607,7 +1607,8 @@ void pauli_async_steps(const ffx_pauli_gate* gates, const std::vector<Step>& ste
                 d.topk_xw = d.trunc_xw;
                 d.trunc_xw = 0;
             }
-            // batched runs compact at every batch end; per-gate runs every 16th truncation
+            // batched runs compact at every batch end; per-gate runs at every compact_every()-th
+            // truncation point (FFX_COMPACT_EVERY, default 256)
             if (s.trunc) d.compact_after = (o.batched || (++trunc_points % every) == 0) ? 1 : 0;
         } else {
             d.kind = 1;

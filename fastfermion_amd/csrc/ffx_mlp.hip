@@ -2,7 +2,7 @@
 // and its C ABI (ffx_pruner_*).
 //
 // Model: MLP 64 -> 128 -> 128 -> 1 with ReLU, bf16 weights and activations,
-// fp32 accumulation on MFMA (v_mfma_f32_16x16x16_bf16); fp32 master
+// fp32 accumulation on MFMA (v_mfma_f32_16x16x32_bf16); fp32 master
 // parameters and Adam moments.  One training step on a minibatch is
 //   k_mlp_fb     forward and backward of every 16-row tile (one wave each):
 //                activations h1, dz1, dz2 (bf16) for the weight gradients,
@@ -133,10 +133,11 @@ static __global__ void __launch_bounds__(kMlpBlock) k_mlp_fb(MlpDev m, const uin
 #pragma unroll
             for (int mt = 0; mt < 8; ++mt) {
                 v4f acc = {0.f, 0.f, 0.f, 0.f};
+                const uint16_t* w = sw2 + (16 * mt + r) * kS2;
 #pragma unroll
-                for (int ks = 0; ks < 8; ++ks)
-                    acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw2 + (16 * mt + r) * kS2 + 16 * ks + 4 * g),
-                                                                    dz2b[ks], acc, 0, 0, 0);
+                for (int s = 0; s < 4; ++s)
+                    acc = mfma32(ld4(w + 32 * s + 4 * g), ld4(w + 32 * s + 16 + 4 * g), dz2b[2 * s], dz2b[2 * s + 1],
+                                 acc);
                 float d[4], h[4];
 #pragma unroll
                 for (int i = 0; i < 4; ++i) {
@@ -168,66 +169,72 @@ static __global__ void __launch_bounds__(kMlpBlock) k_mlp_fb(MlpDev m, const uin
 }
 
 // out[a][b] += sum over rows of A[row][a] * B[row][b] (A rows gathered through aidx
-// when given); a, b multiples of 16, at most 128.  Each block takes a range of
-// rows in chunks of 64 staged in LDS; each wave owns a quarter of the 16x16
-// output tiles.
+// when given); a in {64, 128}, b = 128.  Block (x, y) takes rows [x * rows_per_block,
+// ...) and the output rows [32y, 32y + 32): 2 x 8 tiles of 16x16, four per wave.  The
+// block stages its row chunks (64 rows) of A's 32 columns and of B in LDS with
+// 16-byte loads.
 static __global__ void __launch_bounds__(kMlpBlock) k_wgrad(const uint16_t* __restrict__ A,
                                                             const int32_t* __restrict__ aidx, int a,
                                                             const uint16_t* __restrict__ B, int b, int nrows,
                                                             int rows_per_block, float* __restrict__ out) {
     constexpr int CH = 64;
-    __shared__ __attribute__((aligned(16))) uint16_t sa[CH * (kHid + 8)];
+    constexpr int sA = 32 + 8;
+    __shared__ __attribute__((aligned(16))) uint16_t sa[CH * sA];
     __shared__ __attribute__((aligned(16))) uint16_t sb[CH * (kHid + 8)];
-    const int sA = a + 8, sB = b + 8;
-    const int ta = a / 16, tb = b / 16, ntile = ta * tb;
+    const int sB = b + 8;
+    const int tg = blockIdx.y;  // output rows [32 tg, 32 tg + 32)
     const int wave = threadIdx.x >> 6, l = __lane_id();
     const int r = l & 15, g = l >> 4;
-    v4f acc[16];
+    const int ia = wave >> 1;        // local 16-row tile of the output rows
+    const int ib0 = 4 * (wave & 1);  // first of the wave's four 16-column tiles
+    v4f acc[4];
 #pragma unroll
-    for (int j = 0; j < 16; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
+    for (int j = 0; j < 4; ++j) acc[j] = v4f{0.f, 0.f, 0.f, 0.f};
     const int r0 = blockIdx.x * rows_per_block;
     const int r1 = min(nrows, r0 + rows_per_block);
+    const int bq = b / 8;  // 16-byte pieces of a B row
     for (int c0 = r0; c0 < r1; c0 += CH) {
-        for (int t = threadIdx.x; t < CH * a; t += blockDim.x) {
-            const int rr = t / a, col = t % a, row = c0 + rr;
-            uint16_t v = 0;
-            if (row < r1) v = A[(size_t)(aidx ? aidx[row] : row) * a + col];
-            sa[rr * sA + col] = v;
+        {
+            const int rr = threadIdx.x >> 2, q = threadIdx.x & 3, row = c0 + rr;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (row < r1) v = *(const uint4*)(A + (size_t)(aidx ? aidx[row] : row) * a + 32 * tg + 8 * q);
+            *(uint4*)(sa + rr * sA + 8 * q) = v;
         }
-        for (int t = threadIdx.x; t < CH * b; t += blockDim.x) {
-            const int rr = t / b, col = t % b, row = c0 + rr;
-            sb[rr * sB + col] = row < r1 ? B[(size_t)row * b + col] : (uint16_t)0;
+        for (int t = threadIdx.x; t < CH * bq; t += blockDim.x) {
+            const int rr = t / bq, q = t % bq, row = c0 + rr;
+            uint4 v = {0u, 0u, 0u, 0u};
+            if (row < r1) v = *(const uint4*)(B + (size_t)row * b + 8 * q);
+            *(uint4*)(sb + rr * sB + 8 * q) = v;
         }
         __syncthreads();
 #pragma unroll
-        for (int j = 0; j < 16; ++j) {
-            const int tile = wave + 4 * j;
-            if (tile < ntile) {
-                const int ia = tile / tb, ib = tile % tb;
+        for (int s = 0; s < CH / 32; ++s) {
+            v4s alo, ahi;
 #pragma unroll
-                for (int ks = 0; ks < CH / 16; ++ks) {
-                    v4s av, bv;
+            for (int q = 0; q < 4; ++q) {
+                const int rr = 32 * s + 4 * g + q;  // and rr + 16 for the hi halves
+                alo[q] = (short)sa[rr * sA + 16 * ia + r];
+                ahi[q] = (short)sa[(rr + 16) * sA + 16 * ia + r];
+            }
 #pragma unroll
-                    for (int q = 0; q < 4; ++q) {
-                        const int rr = 16 * ks + 4 * g + q;
-                        av[q] = (short)sa[rr * sA + 16 * ia + r];
-                        bv[q] = (short)sb[rr * sB + 16 * ib + r];
-                    }
-                    acc[j] = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(av, bv, acc[j], 0, 0, 0);
+            for (int j = 0; j < 4; ++j) {
+                v4s blo, bhi;
+#pragma unroll
+                for (int q = 0; q < 4; ++q) {
+                    const int rr = 32 * s + 4 * g + q;
+                    blo[q] = (short)sb[rr * sB + 16 * (ib0 + j) + r];
+                    bhi[q] = (short)sb[(rr + 16) * sB + 16 * (ib0 + j) + r];
                 }
+                acc[j] = mfma32(alo, ahi, blo, bhi, acc[j]);
             }
         }
         __syncthreads();
     }
 #pragma unroll
-    for (int j = 0; j < 16; ++j) {
-        const int tile = wave + 4 * j;
-        if (tile < ntile) {
-            const int ia = tile / tb, ib = tile % tb;
+    for (int j = 0; j < 4; ++j)
 #pragma unroll
-            for (int i = 0; i < 4; ++i) atomicAdd(&out[(size_t)(16 * ia + 4 * g + i) * b + 16 * ib + r], acc[j][i]);
-        }
-    }
+        for (int i = 0; i < 4; ++i)
+            atomicAdd(&out[(size_t)(32 * tg + 16 * ia + 4 * g + i) * b + 16 * (ib0 + j) + r], acc[j][i]);
 }
 
 static __global__ void k_adam(float* __restrict__ p, const float* __restrict__ gr, float* __restrict__ m1,
@@ -321,13 +328,13 @@ int mlp_grad_step(ffx_ctx* ctx, const int32_t* idx, int64_t nb, float under_w) {
     hipLaunchKernelGGL(k_mlp_fb, dim3(fb_grid(ctx, nb)), dim3(kMlpBlock), 0, ctx->stream, s->dev(), s->X, idx, s->T,
                        (int)nb, 1.f / (float)nb, under_w, 1, s->Y, s->H1, s->DZ1, s->DZ2, s->g, s->loss);
     HIPCHK(hipGetLastError());
-    const int rpb = 256;
+    const int rpb = 128;
     const unsigned gw = (unsigned)((nb + rpb - 1) / rpb);
-    hipLaunchKernelGGL(k_wgrad, dim3(gw), dim3(kMlpBlock), 0, ctx->stream, s->X, idx, kFeat, s->DZ1, kHid, (int)nb,
-                       rpb, s->g + kOffW1);
+    hipLaunchKernelGGL(k_wgrad, dim3(gw, kFeat / 32), dim3(kMlpBlock), 0, ctx->stream, s->X, idx, kFeat, s->DZ1, kHid,
+                       (int)nb, rpb, s->g + kOffW1);
     HIPCHK(hipGetLastError());
-    hipLaunchKernelGGL(k_wgrad, dim3(gw), dim3(kMlpBlock), 0, ctx->stream, s->H1, (const int32_t*)nullptr, kHid,
-                       s->DZ2, kHid, (int)nb, rpb, s->g + kOffW2);
+    hipLaunchKernelGGL(k_wgrad, dim3(gw, kHid / 32), dim3(kMlpBlock), 0, ctx->stream, s->H1, (const int32_t*)nullptr,
+                       kHid, s->DZ2, kHid, (int)nb, rpb, s->g + kOffW2);
     HIPCHK(hipGetLastError());
     return FFX_OK;
 }

@@ -21,7 +21,7 @@
 // step, which must equal the final expectation (an exact invariant).
 //
 // MLP: 64 -> 128 -> 128 -> 1, ReLU, bf16 weights and activations, fp32
-// accumulation on MFMA (v_mfma_f32_16x16x16_bf16).  Layers are evaluated in
+// accumulation on MFMA (v_mfma_f32_16x16x32_bf16).  Layers are evaluated in
 // transposed form (hidden x terms), so the accumulator layout of one layer is
 // the B-operand layout of the next and no LDS round trip is needed between them.
 #pragma once
@@ -335,6 +335,24 @@ constexpr int kS2 = kHid + 8;   // W2^T / W2 rows
 
 __device__ __forceinline__ v4s ld4(const uint16_t* p) { return *(const v4s*)p; }
 
+typedef short v8s __attribute__((ext_vector_type(8)));
+typedef __bf16 v8bf __attribute__((ext_vector_type(8)));
+
+// One v_mfma_f32_16x16x32_bf16 (gfx950) over a 32-long slice of the
+// contraction.  Operand slots 0-3 of lane group g (= lane / 16) carry slice
+// positions 4g .. 4g+3 (the `lo` halves), slots 4-7 positions 16+4g .. 16+4g+3
+// (`hi`).  A and B use the same slot map, so the instruction sums over all 32
+// positions, whatever order the hardware adds them in.  In this map the two
+// 16x16 accumulator tiles of a layer's outputs 32s .. 32s+15 and 32s+16 ..
+// 32s+31 (output 16t + 4g + i in slot i of lane group g) are directly the B
+// operand of slice s of the next layer.
+__device__ __forceinline__ v4f mfma32(v4s a_lo, v4s a_hi, v4s b_lo, v4s b_hi, v4f acc) {
+    const v8s a = __builtin_shufflevector(a_lo, a_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    const v8s b = __builtin_shufflevector(b_lo, b_hi, 0, 1, 2, 3, 4, 5, 6, 7);
+    return __builtin_amdgcn_mfma_f32_16x16x32_bf16(__builtin_bit_cast(v8bf, a), __builtin_bit_cast(v8bf, b), acc, 0,
+                                                   0, 0);
+}
+
 __device__ __forceinline__ v4s pack4(float a, float b, float c, float d) {
     v4s r;
     r[0] = (short)f2bf(a);
@@ -360,10 +378,10 @@ __device__ __forceinline__ float mlp16_forward(const uint16_t* sw1t, const uint1
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
         v4f acc = {0.f, 0.f, 0.f, 0.f};
+        const uint16_t* w = sw1t + (16 * mt + r) * kS1;
 #pragma unroll
-        for (int ks = 0; ks < 4; ++ks)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw1t + (16 * mt + r) * kS1 + 16 * ks + 4 * g), xb[ks],
-                                                            acc, 0, 0, 0);
+        for (int s = 0; s < 2; ++s)
+            acc = mfma32(ld4(w + 32 * s + 4 * g), ld4(w + 32 * s + 16 + 4 * g), xb[2 * s], xb[2 * s + 1], acc);
 #pragma unroll
         for (int i = 0; i < 4; ++i) acc[i] += sb1[16 * mt + 4 * g + i];
         z1[mt] = acc;
@@ -377,10 +395,10 @@ __device__ __forceinline__ float mlp16_forward(const uint16_t* sw1t, const uint1
 #pragma unroll
     for (int mt = 0; mt < 8; ++mt) {
         v4f acc = {0.f, 0.f, 0.f, 0.f};
+        const uint16_t* w = sw2t + (16 * mt + r) * kS2;
 #pragma unroll
-        for (int ks = 0; ks < 8; ++ks)
-            acc = __builtin_amdgcn_mfma_f32_16x16x16bf16_1k(ld4(sw2t + (16 * mt + r) * kS2 + 16 * ks + 4 * g), h1[ks],
-                                                            acc, 0, 0, 0);
+        for (int s = 0; s < 4; ++s)
+            acc = mfma32(ld4(w + 32 * s + 4 * g), ld4(w + 32 * s + 16 + 4 * g), h1[2 * s], h1[2 * s + 1], acc);
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int o = 16 * mt + 4 * g + i;
