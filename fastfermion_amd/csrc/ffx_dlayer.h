@@ -1,0 +1,569 @@
+// ffx_dlayer.h -- a layer of rotations with pairwise disjoint supports in one
+// launch (Pauli axes on disjoint qubit sets, Majorana strings of even degree on
+// disjoint index sets; e.g. one bond matching or the on-site terms of a
+// Fermi-Hubbard Trotter step).
+//
+// Such rotations commute, and flipping a term q by one axis P_j changes neither
+// its commutation with any other axis of the layer (their supports do not meet
+// P_j's) nor with P_j itself (the axes have even overlap parity with themselves).
+// So the set A(q) of layer rotations that anticommute with q is the same for
+// every member of q's orbit q + span{P_j : j in A(q)}; each j in A(q) has a unit
+// (qubit or Majorana index) in q's support, so |A| <= degree(q).  Every axis has
+// a private pivot bit (the lowest bit of its key mask, set in no other axis of
+// the layer): the orbit's canonical member is the one with the pivot bits of
+// its active axes cleared.  Applying the layer gate by gate only mixes values
+// inside an orbit, so one launch can do the whole layer exactly as the layer of
+// single-qubit rotations does (ffx_layer.h): claim the canonical key, one lane
+// group or wave per leader, butterfly replay of the active rotations in layer
+// order with the one-rotation kernel's operations (per-member degree filter and
+// sign), truncation points between stages, staged appends.
+//
+// Requires every live term's degree <= kLayerMaxA (host check as for layers).
+#pragma once
+
+#include "ffx_layer.h"
+
+namespace ffx {
+
+template <class Alg, class CP>
+struct DLayerGate {
+    typename Alg::Gate g;
+    typename CP::fac_t cs, sn;
+    int32_t pivot;   // private pivot bit (key bit index, word pivot / 64)
+    int32_t thresh;  // threshold truncation after this rotation
+    int32_t xw;      // X/Y-weight cap after this rotation
+    int32_t pad;
+};
+
+template <int W, class CP, class Alg>
+struct DLayer {
+    const DLayerGate<Alg, CP>* gates;  // nr records in layer order
+    const int16_t* pos;                // unit -> position in the layer, -1: none (256 entries)
+    int32_t* sd;                       // live-count change at each rotation (nr entries), kStripes copies
+    uint32_t sd_stride;
+    uint64_t* claim;                   // claim table (hmask + 1 entries)
+    uint64_t umask[W];                 // units covered by the layer's axes
+    uint64_t tmask[4], xmask[4];       // layer positions followed by the threshold / by the X/Y cap
+    double mincoeff;
+    int nr;
+    int maxdeg;
+    int max_xw;
+    uint32_t stamp;  // this launch's stamp (24 bits, nonzero)
+};
+
+// Units of a key (qubits in the Pauli support, Majorana indices), word w.
+template <class Alg>
+__device__ __forceinline__ uint64_t key_units(const uint64_t (&k)[Alg::NP], int w) {
+    if constexpr (Alg::kPauli) {
+        return k[w] | k[Alg::W + w];
+    } else {
+        return k[w];
+    }
+}
+
+// Positions of the layer rotations that anticommute with key k (a mask of
+// layer positions); *na receives their number.
+template <class Alg, class CP>
+__device__ __forceinline__ void dlayer_active(const uint64_t (&k)[Alg::NP], const uint64_t* umask,
+                                              const int16_t* spos, const DLayerGate<Alg, CP>* sg, uint64_t (&pm)[4],
+                                              int& na) {
+    constexpr int W = Alg::W;
+    uint64_t cand[4] = {0, 0, 0, 0};
+#pragma unroll
+    for (int w = 0; w < W; ++w) {
+        uint64_t b = key_units<Alg>(k, w) & umask[w];
+        while (b) {
+            const int t = __builtin_ctzll(b);
+            b &= b - 1;
+            const int p = spos[64 * w + t];
+#pragma unroll
+            for (int u = 0; u < 4; ++u)
+                if (u == (p >> 6)) cand[u] |= 1ull << (p & 63);
+        }
+    }
+    na = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        pm[u] = 0;
+        uint64_t b = cand[u];
+        while (b) {
+            const int t = __builtin_ctzll(b);
+            b &= b - 1;
+            if (Alg::anticommutes(k, sg[64 * u + t].g)) {
+                pm[u] |= 1ull << t;
+                ++na;
+            }
+        }
+    }
+}
+
+// Canonical member: the pivot bit of every active axis cleared.
+template <class Alg, class CP>
+__device__ __forceinline__ void dlayer_canon(const uint64_t (&k)[Alg::NP], const uint64_t (&pm)[4],
+                                             const DLayerGate<Alg, CP>* sg, uint64_t (&ck)[Alg::NP]) {
+    constexpr int NP = Alg::NP;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) ck[q] = k[q];
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        uint64_t b = pm[u];
+        while (b) {
+            const int t = __builtin_ctzll(b);
+            b &= b - 1;
+            const DLayerGate<Alg, CP>& lg = sg[64 * u + t];
+            const int pv = lg.pivot;
+            bool set = false;
+#pragma unroll
+            for (int q = 0; q < NP; ++q)
+                if (q == (pv >> 6)) set = ((ck[q] >> (pv & 63)) & 1) != 0;
+            if (set)
+#pragma unroll
+                for (int q = 0; q < NP; ++q) ck[q] ^= lg.g.p[q];
+        }
+    }
+}
+
+// Member T of the orbit of canonical key ck: the axes of the active positions
+// whose bit d (d-th active position in layer order) is set in T.
+template <class Alg, class CP>
+__device__ __forceinline__ void dlayer_member(const uint64_t (&ck)[Alg::NP], const uint64_t (&pm)[4], int a, int T,
+                                              const DLayerGate<Alg, CP>* sg, uint64_t (&key)[Alg::NP]) {
+    constexpr int NP = Alg::NP;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) key[q] = ck[q];
+    uint64_t pw[4] = {pm[0], pm[1], pm[2], pm[3]};
+    for (int d = 0; d < a; ++d) {
+        const int pq = pop_pos(pw, 0);
+        if ((T >> d) & 1)
+#pragma unroll
+            for (int q = 0; q < NP; ++q) key[q] ^= sg[pq].g.p[q];
+    }
+}
+
+template <class Alg, class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const typename CP::store_t* __restrict__ cin,
+                                                         typename CP::store_t* __restrict__ cout,
+                                                         DLayer<Alg::W, CP, Alg> L, Epochs ep, uint32_t e) {
+    constexpr int NP = Alg::NP;
+    using val_t = typename CP::val_t;
+    __shared__ int16_t spos[256];
+    __shared__ DLayerGate<Alg, CP> sg[kLayerMax];
+    __shared__ uint64_t skey[kLayerStage][NP];
+    __shared__ val_t sval[kLayerStage][K];
+    __shared__ int32_t ssd[kLayerMax];
+    __shared__ uint32_t s_cnt, s_base;
+    // leaders of the block iteration, by number of active axes (1..8): their keys and
+    // active position masks, so that any wave of the block can replay any leader's orbit
+    __shared__ uint64_t slk[kBlock][NP];
+    __shared__ uint64_t spm[kBlock][4];
+    __shared__ uint16_t slist[kLayerMaxA + 1][kBlock];
+    __shared__ uint32_t scount[kLayerMaxA + 1];
+    uint32_t n, dead;
+    if (!step_enter(ep, e, n, dead)) return;
+    // slots per block iteration: the fewest of 4, 16, 64 that fit every slot into the
+    // grid, else one per thread (as in the single-qubit layer kernel)
+    uint32_t tile = blockDim.x;
+    for (uint32_t t = 4; t < blockDim.x; t <<= 2)
+        if ((uint64_t)n <= (uint64_t)t * gridDim.x) {
+            tile = t;
+            break;
+        }
+    const uint32_t step = gridDim.x * tile;
+    if (blockIdx.x * tile >= n) return;  // block-uniform
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) spos[t] = L.pos[t];
+    for (int t = threadIdx.x; t < L.nr; t += blockDim.x) {
+        sg[t] = L.gates[t];
+        ssd[t] = 0;
+    }
+    const size_t cap = st.cap;
+    const int lane = __lane_id();
+    int dd = 0;
+    __syncthreads();
+    const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    // stage a fresh live member for appending (one global reservation per block iteration)
+    auto stage_append = [&](const uint64_t (&key)[NP], const val_t (&v)[K]) {
+        const uint32_t r = atomicAdd(&s_cnt, 1u);
+        if (r < (uint32_t)kLayerStage) {
+#pragma unroll
+            for (int q = 0; q < NP; ++q) skey[r][q] = key[q];
+#pragma unroll
+            for (int kk = 0; kk < K; ++kk) sval[r][kk] = v[kk];
+        } else {
+            const uint64_t at = (uint64_t)n + atomicAdd(&ep.app[e], 1u);
+            if (at < cap) {
+                put_slot<NP, CP, K>(st, st.key, cout, at, key, v);
+            } else {
+                atomicOr(&ep.flags[0], 1u);
+                ep.flags[1] = e;
+            }
+        }
+    };
+    for (uint32_t b0 = blockIdx.x * tile; b0 < n; b0 += step) {
+        if (threadIdx.x == 0) s_cnt = 0;
+        if (threadIdx.x <= kLayerMaxA) scount[threadIdx.x] = 0;
+        __syncthreads();
+        // ---- phase 1: own slot: copy if inactive, else claim the orbit ----
+        const uint32_t i = b0 + threadIdx.x;
+        if (threadIdx.x < tile && i < n) {
+            uint64_t k[NP], pm[4];
+            key_load<NP>(st.key, i, k);
+            int a = 0;
+            dlayer_active<Alg, CP>(k, L.umask, spos, sg, pm, a);
+            if (a == 0) {
+                // untouched by every rotation of the layer; the truncation points inside
+                // the layer still apply (to an unchanged value: once is enough)
+                val_t c[K];
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) c[kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, i);
+                bool kill = false;
+                int at = -1;
+                if (!CP::dead(c[0])) {
+                    const int pt = first_in(L.tmask, 0, L.nr), px = first_in(L.xmask, 0, L.nr);
+                    if (pt >= 0 && magnitude<CP, K>(c) <= L.mincoeff) {
+                        kill = true;
+                        at = pt;
+                    }
+                    if (px >= 0 && Alg::xweight(k) > L.max_xw && (!kill || px < at)) {
+                        kill = true;
+                        at = px;
+                    }
+                }
+                if (kill) {
+                    c[0] = CP::make_dead();
+#pragma unroll
+                    for (int kk = 1; kk < K; ++kk) c[kk] = CP::from_double(0.0);
+                    ++dd;
+                    atomicAdd(&ssd[at], -1);
+                }
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) CP::store(cout + (size_t)kk * CP::NS * cap, cap, i, c[kk]);
+            } else if (a > kLayerMaxA) {
+                // more active axes than a live term can have: its orbit holds no live
+                // member; copy it through (a live one is an internal error)
+                const val_t c0 = CP::load(cin, cap, i);
+                if (!CP::dead(c0)) atomicOr(&st.ctrl->index_full, 4u);
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk)
+                    CP::store(cout + (size_t)kk * CP::NS * cap, cap, i, CP::load(cin + (size_t)kk * CP::NS * cap, cap, i));
+            } else {
+                uint64_t ck[NP];
+                dlayer_canon<Alg, CP>(k, pm, sg, ck);
+                const uint64_t h = hmix(lin_hash<NP>(ck));
+                const uint64_t tag8 = (h >> 56) & 0xFFull;
+                const uint64_t mine = ((uint64_t)L.stamp << 40) | (tag8 << 32) | (uint64_t)i;
+                uint64_t p = h & st.hmask;
+                bool leader = false;
+                for (uint64_t s = 0;; ++s) {
+                    if (s > st.hmask) {
+                        atomicOr(&st.ctrl->index_full, 2u);
+                        break;
+                    }
+                    uint64_t e0 = L.claim[p];
+                    if ((e0 >> 40) != L.stamp) {
+                        const uint64_t old = (uint64_t)atomicCAS((unsigned long long*)&L.claim[p],
+                                                                 (unsigned long long)e0, (unsigned long long)mine);
+                        if (old == e0) {
+                            leader = true;
+                            break;
+                        }
+                        e0 = old;
+                        if ((e0 >> 40) != L.stamp) continue;  // overwritten meanwhile: look again
+                    }
+                    if (((e0 >> 32) & 0xFFull) == tag8) {
+                        const uint32_t j = (uint32_t)e0;
+                        uint64_t kj[NP], pmj[4], ckj[NP];
+                        int aj = 0;
+                        key_load<NP>(st.key, j, kj);
+                        dlayer_active<Alg, CP>(kj, L.umask, spos, sg, pmj, aj);
+                        dlayer_canon<Alg, CP>(kj, pmj, sg, ckj);
+                        bool eq = true;
+#pragma unroll
+                        for (int q = 0; q < NP; ++q) eq = eq && ckj[q] == ck[q];
+                        if (eq) break;  // another slot leads this orbit
+                    }
+                    p = (p + 1) & st.hmask;
+                }
+                if (leader) {
+#pragma unroll
+                    for (int q = 0; q < NP; ++q) slk[threadIdx.x][q] = ck[q];
+#pragma unroll
+                    for (int u = 0; u < 4; ++u) spm[threadIdx.x][u] = pm[u];
+                    slist[a][atomicAdd(&scount[a], 1u)] = (uint16_t)threadIdx.x;
+                }
+            }
+        }
+        __syncthreads();
+        // ---- phase 2a: orbits with <= 6 active axes, 64 / 2^A of them per wave pass ----
+        int poff = 0;
+        for (int A = 1; A <= 6; ++A) {
+            const int nlead = (int)scount[A], per = 64 >> A;
+            const int npass = (nlead + per - 1) / per;
+            const int pass0 = ((wave - poff) % nwave + nwave) % nwave;
+            poff += npass;
+            for (int pass = pass0; pass < npass; pass += nwave) {
+                const int g = lane >> A, T = lane & ((1 << A) - 1);
+                const int kth = pass * per + g;
+                const bool on = kth < nlead;
+                const int src = on ? (int)slist[A][kth] : 0;
+                uint64_t ck[NP], pm[4], key[NP];
+#pragma unroll
+                for (int q = 0; q < NP; ++q) ck[q] = on ? slk[src][q] : 0ull;
+#pragma unroll
+                for (int u = 0; u < 4; ++u) pm[u] = on ? spm[src][u] : 0ull;
+                dlayer_member<Alg, CP>(ck, pm, on ? A : 0, T, sg, key);
+                const bool recv_ok = Alg::degree(key) <= L.maxdeg;
+                bool exist = false, alive = false;
+                uint32_t slot = kNone;
+                val_t v[K];
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) v[kk] = CP::from_double(0.0);
+                if (on) {
+                    slot = probe<NP>(st.key, cap, st.index, st.hmask, key, n, &st.ctrl->index_full);
+                    if (slot != kNone) {
+                        exist = true;
+#pragma unroll
+                        for (int kk = 0; kk < K; ++kk) v[kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, slot);
+                        alive = !CP::dead(v[0]);
+                    }
+                }
+                const bool alive0 = alive;
+                // truncation points in [lo, hi) on this lane's member
+                auto trunc1 = [&](int lo, int hi) {
+                    if (!alive) return;
+                    const int pt = first_in(L.tmask, lo, hi), px = first_in(L.xmask, lo, hi);
+                    const bool bt = pt >= 0 && magnitude<CP, K>(v) <= L.mincoeff;
+                    const bool bx = px >= 0 && Alg::xweight(key) > L.max_xw;
+                    if (bt && (!bx || pt <= px)) {
+                        alive = false;
+                        atomicAdd(&ssd[pt], -1);
+                    } else if (bx) {
+                        alive = false;
+                        atomicAdd(&ssd[px], -1);
+                    }
+                };
+                uint64_t pw[4] = {pm[0], pm[1], pm[2], pm[3]};
+                trunc1(0, peek_pos(pw, L.nr));
+                for (int d = 0; d < A; ++d) {
+                    const int pq = pop_pos(pw, L.nr);
+                    const DLayerGate<Alg, CP>& lg = sg[on ? pq : 0];
+                    val_t vu[K];
+#pragma unroll
+                    for (int kk = 0; kk < K; ++kk) vu[kk] = shfl_xor_val<CP>(v[kk], 1 << d);
+                    const bool au = __shfl_xor((int)alive, 1 << d, 64) != 0;
+                    if (on) {
+                        const bool before = alive;
+                        const bool recv = au && recv_ok;
+                        if (alive || recv) {
+                            uint64_t ku[NP];
+#pragma unroll
+                            for (int qq = 0; qq < NP; ++qq) ku[qq] = key[qq] ^ lg.g.p[qq];
+                            const int s_in = recv ? Alg::sign(ku, key, lg.g) : 1;
+#pragma unroll
+                            for (int kk = 0; kk < K; ++kk) {
+                                if (alive) {
+                                    val_t o = CP::mul(v[kk], lg.cs);
+                                    if (recv) o = CP::add(o, CP::mul_signed(vu[kk], lg.sn, s_in));
+                                    v[kk] = o;
+                                } else {
+                                    v[kk] = CP::mul_signed(vu[kk], lg.sn, s_in);
+                                }
+                            }
+                            alive = true;
+                        }
+                        if (alive != before) atomicAdd(&ssd[pq], alive ? 1 : -1);
+                        trunc1(pq, peek_pos(pw, L.nr));
+                    }
+                }
+                if (on) {
+                    if (exist) {
+                        if (alive) {
+#pragma unroll
+                            for (int kk = 0; kk < K; ++kk) CP::store(cout + (size_t)kk * CP::NS * cap, cap, slot, v[kk]);
+                        } else {
+                            CP::store(cout, cap, slot, CP::make_dead());
+#pragma unroll
+                            for (int kk = 1; kk < K; ++kk)
+                                CP::store(cout + (size_t)kk * CP::NS * cap, cap, slot, CP::from_double(0.0));
+                        }
+                        dd += (int)alive0 - (int)alive;
+                    } else if (alive) {
+                        stage_append(key, v);
+                    }
+                }
+            }
+        }
+        // ---- phase 2b: orbits with 7 or 8 active axes, one per wave pass ----
+        const int n7 = (int)scount[7], n78 = n7 + (int)scount[8];
+        for (int jb = ((wave - poff) % nwave + nwave) % nwave; jb < n78; jb += nwave) {
+            const int src = jb < n7 ? (int)slist[7][jb] : (int)slist[8][jb - n7];
+            const int a = jb < n7 ? 7 : 8;
+            uint64_t ck[NP], pm[4];
+#pragma unroll
+            for (int q = 0; q < NP; ++q) ck[q] = slk[src][q];
+#pragma unroll
+            for (int u = 0; u < 4; ++u) pm[u] = spm[src][u];
+            const int nm = 1 << a;
+            // members T = lane + 64 m
+            uint64_t key[4][NP];
+            val_t v[4][K];
+            uint32_t slot[4];
+            bool alive[4], exist[4], recv_ok[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int T = lane + 64 * m;
+                exist[m] = alive[m] = false;
+                slot[m] = kNone;
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) v[m][kk] = CP::from_double(0.0);
+                dlayer_member<Alg, CP>(ck, pm, T < nm ? a : 0, T, sg, key[m]);
+                recv_ok[m] = Alg::degree(key[m]) <= L.maxdeg;
+                if (T < nm) {
+                    slot[m] = probe<NP>(st.key, cap, st.index, st.hmask, key[m], n, &st.ctrl->index_full);
+                    if (slot[m] != kNone) {
+                        exist[m] = true;
+#pragma unroll
+                        for (int kk = 0; kk < K; ++kk)
+                            v[m][kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, slot[m]);
+                        alive[m] = !CP::dead(v[m][0]);
+                    }
+                }
+            }
+            bool alive0[4];
+#pragma unroll
+            for (int m = 0; m < 4; ++m) alive0[m] = alive[m];
+            // truncation points in layer positions [lo, hi) applied to the orbit's current
+            // values; erasures are counted at the first such point that erases
+            auto truncate_range = [&](int lo, int hi) {
+                const int pt = first_in(L.tmask, lo, hi), px = first_in(L.xmask, lo, hi);
+                if (pt < 0 && px < 0) return;
+                int kt = 0, kx = 0;
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (!alive[m]) continue;
+                    const bool bt = pt >= 0 && magnitude<CP, K>(v[m]) <= L.mincoeff;
+                    const bool bx = px >= 0 && Alg::xweight(key[m]) > L.max_xw;
+                    if (bt && (!bx || pt <= px)) {
+                        alive[m] = false;
+                        ++kt;
+                    } else if (bx) {
+                        alive[m] = false;
+                        ++kx;
+                    }
+                }
+                for (int o = 32; o > 0; o >>= 1) {
+                    kt += __shfl_xor(kt, o, 64);
+                    kx += __shfl_xor(kx, o, 64);
+                }
+                if (lane == 0) {
+                    if (kt) atomicAdd(&ssd[pt], -kt);
+                    if (kx) atomicAdd(&ssd[px], -kx);
+                }
+            };
+            uint64_t pw[4] = {pm[0], pm[1], pm[2], pm[3]};
+            truncate_range(0, peek_pos(pw, L.nr));
+            for (int d = 0; d < a; ++d) {
+                const int pq = pop_pos(pw, L.nr);
+                const DLayerGate<Alg, CP>& lg = sg[pq];
+                const int bitd = 1 << d;
+                int before = 0, after = 0;
+                val_t vu[4][K];
+                bool au[4];
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    if (bitd < 64) {
+#pragma unroll
+                        for (int kk = 0; kk < K; ++kk) vu[m][kk] = shfl_xor_val<CP>(v[m][kk], bitd);
+                        au[m] = __shfl_xor((int)alive[m], bitd, 64) != 0;
+                    } else if (bitd == 64) {
+#pragma unroll
+                        for (int kk = 0; kk < K; ++kk) vu[m][kk] = v[m ^ 1][kk];
+                        au[m] = alive[m ^ 1];
+                    } else {
+#pragma unroll
+                        for (int kk = 0; kk < K; ++kk) vu[m][kk] = v[m ^ 2][kk];
+                        au[m] = alive[m ^ 2];
+                    }
+                }
+#pragma unroll
+                for (int m = 0; m < 4; ++m) {
+                    const int T = lane + 64 * m;
+                    if (T >= nm) continue;
+                    before += alive[m];
+                    const bool recv = au[m] && recv_ok[m];
+                    if (alive[m] || recv) {
+                        uint64_t ku[NP];
+#pragma unroll
+                        for (int qq = 0; qq < NP; ++qq) ku[qq] = key[m][qq] ^ lg.g.p[qq];
+                        const int s_in = recv ? Alg::sign(ku, key[m], lg.g) : 1;
+#pragma unroll
+                        for (int kk = 0; kk < K; ++kk) {
+                            if (alive[m]) {
+                                val_t o = CP::mul(v[m][kk], lg.cs);
+                                if (recv) o = CP::add(o, CP::mul_signed(vu[m][kk], lg.sn, s_in));
+                                v[m][kk] = o;
+                            } else {
+                                v[m][kk] = CP::mul_signed(vu[m][kk], lg.sn, s_in);
+                            }
+                        }
+                        alive[m] = true;
+                    }
+                    after += alive[m];
+                }
+                // live-count change of the orbit from this rotation, then its truncation points
+                int delta = after - before;
+                for (int o = 32; o > 0; o >>= 1) delta += __shfl_xor(delta, o, 64);
+                if (lane == 0 && delta) atomicAdd(&ssd[pq], delta);
+                truncate_range(pq, peek_pos(pw, L.nr));
+            }
+            // write back existing members, stage fresh live ones
+#pragma unroll
+            for (int m = 0; m < 4; ++m) {
+                const int T = lane + 64 * m;
+                if (T >= nm) continue;
+                if (exist[m]) {
+                    if (alive[m]) {
+#pragma unroll
+                        for (int kk = 0; kk < K; ++kk)
+                            CP::store(cout + (size_t)kk * CP::NS * cap, cap, slot[m], v[m][kk]);
+                    } else {
+                        CP::store(cout, cap, slot[m], CP::make_dead());
+#pragma unroll
+                        for (int kk = 1; kk < K; ++kk)
+                            CP::store(cout + (size_t)kk * CP::NS * cap, cap, slot[m], CP::from_double(0.0));
+                    }
+                    dd += (int)alive0[m] - (int)alive[m];
+                } else if (alive[m]) {
+                    stage_append(key[m], v[m]);
+                }
+            }
+        }
+        __syncthreads();
+        // ---- phase 3: one reservation per block for the staged appends ----
+        const uint32_t staged = min(s_cnt, (uint32_t)kLayerStage);
+        if (threadIdx.x == 0 && staged) s_base = atomicAdd(&ep.app[e], staged);
+        __syncthreads();
+        for (uint32_t r = threadIdx.x; r < staged; r += blockDim.x) {
+            const uint64_t at = (uint64_t)n + s_base + r;
+            if (at < cap) {
+                uint64_t kr[NP];
+                val_t vr[K];
+#pragma unroll
+                for (int q = 0; q < NP; ++q) kr[q] = skey[r][q];
+#pragma unroll
+                for (int kk = 0; kk < K; ++kk) vr[kk] = sval[r][kk];
+                put_slot<NP, CP, K>(st, st.key, cout, at, kr, vr);
+            } else {
+                atomicOr(&ep.flags[0], 1u);
+                ep.flags[1] = e;
+            }
+        }
+        __syncthreads();
+    }
+    dd = block_sum_int(dd);
+    if (threadIdx.x == 0 && dd) atomicAdd(&ep.dd[e], dd);
+    int32_t* sd = L.sd + (size_t)(blockIdx.x % kStripes) * L.sd_stride;
+    for (int t = threadIdx.x; t < L.nr; t += blockDim.x)
+        if (ssd[t]) atomicAdd(&sd[t], ssd[t]);
+}
+
+}  // namespace ffx

@@ -28,6 +28,7 @@
 #include "ffx_batch.h"
 #include "ffx_layer.h"
 #include "ffx_prune.h"
+#include "ffx_dlayer.h"
 
 using namespace ffx;
 
@@ -913,6 +914,39 @@ struct Engine {
         return k >= (size_t)kLayerMin ? k : 0;
     }
 
+    // Units of a rotation axis (qubits of a Pauli axis, indices of a Majorana string).
+    static void axis_units(const typename Alg::Gate& g, uint64_t (&u)[4]) {
+        constexpr int W = Alg::W;
+        for (int w = 0; w < 4; ++w) u[w] = 0;
+        for (int w = 0; w < W; ++w) u[w] = Alg::kPauli ? (g.p[w] | g.p[(W + w) % NP]) : g.p[w];
+    }
+
+    // Disjoint-support layer starting at rotation step t: consecutive rotations
+    // whose axes have pairwise disjoint supports (Majorana strings of even
+    // degree), at most kLayerMax; top-k and pruning points end it, compaction
+    // points inside move to its end; 0 when shorter than kLayerMin.
+    static size_t plan_dlayer(const std::vector<AStep>& steps, size_t t) {
+        uint64_t used[4] = {0, 0, 0, 0};
+        size_t k = 0;
+        while (t + k < steps.size() && k < (size_t)kLayerMax) {
+            const AStep& a = steps[t + k];
+            if (a.kind != 0) break;
+            if (k > 0 && (steps[t + k - 1].topk_after || steps[t + k - 1].prune_after)) break;
+            uint64_t u[4];
+            axis_units(a.g, u);
+            int cnt = 0;
+            bool overlap = false;
+            for (int w = 0; w < 4; ++w) {
+                cnt += __builtin_popcountll(u[w]);
+                overlap = overlap || (u[w] & used[w]) != 0;
+            }
+            if (cnt == 0 || overlap || (!Alg::kPauli && (cnt & 1))) break;
+            for (int w = 0; w < 4; ++w) used[w] |= u[w];
+            ++k;
+        }
+        return k >= (size_t)kLayerMin ? k : 0;
+    }
+
     TopK topk_scratch() const {
         TopK tk;
         tk.hist = ctx->topk_buf;
@@ -990,27 +1024,41 @@ struct Engine {
         if (const char* ev = getenv("FFX_WINDOW")) max_window = std::min(kWin, std::max(1, atoi(ev)));
         // layers of single-qubit rotations (one launch each), planned greedily from the
         // start; overflow restarts happen at launch boundaries, so the plan stays valid
-        std::vector<uint32_t> layer_len(steps.size(), 0), layer_off(steps.size(), 0);
-        size_t layer_bytes = 0, n_layer_rot = 0;
-        if constexpr (Alg::kPauli) {
-            if (layers) {
-                for (size_t t = 0; t < steps.size();) {
-                    const size_t k = plan_layer(steps, t);
-                    if (k) {
-                        layer_len[t] = (uint32_t)k;
-                        layer_off[t] = (uint32_t)n_layer_rot;
-                        n_layer_rot += k;
-                        t += k;
-                    } else {
-                        t += steps[t].kind == 0 ? window_len(steps, t, max_window) : 1;
-                    }
+        // layers: kind 1 (single-qubit Pauli rotations, ffx_layer.h) or kind 2 (disjoint
+        // supports, ffx_dlayer.h); layer_off numbers their rotations for the counters,
+        // rec_off their records (each kind in its own array)
+        std::vector<uint32_t> layer_len(steps.size(), 0), layer_off(steps.size(), 0), rec_off(steps.size(), 0);
+        std::vector<uint8_t> layer_kind(steps.size(), 0);
+        size_t layer_bytes = 0, n_layer_rot = 0, n_rec1 = 0, n_rec2 = 0;
+        if (layers) {
+            const bool dlayers = getenv("FFX_NO_DLAYERS") == nullptr;  // kind 2 off: a cross-check
+            for (size_t t = 0; t < steps.size();) {
+                size_t k = 0;
+                int kind = 0;
+                if constexpr (Alg::kPauli) {
+                    k = plan_layer(steps, t);
+                    kind = k ? 1 : 0;
+                }
+                if (!k && dlayers && (k = plan_dlayer(steps, t)) != 0) kind = 2;
+                if (k) {
+                    layer_len[t] = (uint32_t)k;
+                    layer_off[t] = (uint32_t)n_layer_rot;
+                    layer_kind[t] = (uint8_t)kind;
+                    rec_off[t] = (uint32_t)(kind == 1 ? n_rec1 : n_rec2);
+                    (kind == 1 ? n_rec1 : n_rec2) += k;
+                    n_layer_rot += k;
+                    t += k;
+                } else {
+                    t += steps[t].kind == 0 ? window_len(steps, t, max_window) : 1;
                 }
             }
         }
         size_t n_layers = 0;
         for (size_t t = 0; t < steps.size(); ++t) n_layers += layer_len[t] ? 1 : 0;
-        // buffer: records (n_layer_rot), position tables (256 int16 per layer), counters (n_layer_rot int32)
-        const size_t lrec_bytes = n_layer_rot * sizeof(LayerGate<CP>);
+        // buffer: records (kind 1, then kind 2), position tables (256 int16 per layer),
+        // counters (n_layer_rot int32)
+        const size_t lrec2_off = (n_rec1 * sizeof(LayerGate<CP>) + 255) & ~(size_t)255;
+        const size_t lrec_bytes = lrec2_off + n_rec2 * sizeof(DLayerGate<Alg, CP>);
         const size_t lpos_off = (lrec_bytes + 255) & ~(size_t)255;
         const size_t lsd_off = lpos_off + ((n_layers * 256 * sizeof(int16_t) + 255) & ~(size_t)255);
         const size_t lsd_stride = (n_layer_rot + 63) & ~(size_t)63;  // striped counter copies
@@ -1026,6 +1074,7 @@ struct Engine {
             }
             std::vector<char> hb(lsd_off, 0);
             LayerGate<CP>* rec = (LayerGate<CP>*)hb.data();
+            DLayerGate<Alg, CP>* rec2 = (DLayerGate<Alg, CP>*)(hb.data() + lrec2_off);
             int16_t* pos = (int16_t*)(hb.data() + lpos_off);
             size_t li = 0;
             for (size_t t = 0; t < steps.size(); ++t) {
@@ -1033,11 +1082,37 @@ struct Engine {
                 int16_t* pt = pos + 256 * li;
                 for (int q = 0; q < 256; ++q) pt[q] = -1;
                 uint64_t m3[kLM] = {};
+                if (layer_kind[t] == 2) {
+                    // records with pivots; unit -> position table; union of the units
+                    for (uint32_t j = 0; j < layer_len[t]; ++j) {
+                        const AStep& a = steps[t + j];
+                        DLayerGate<Alg, CP>& r = rec2[rec_off[t] + j];
+                        r.g = a.g;
+                        r.cs = make_fac<CP>(a.cs);
+                        r.sn = make_fac<CP>(a.sn);
+                        r.thresh = a.trunc_thresh;
+                        r.xw = a.trunc_xw;
+                        r.pivot = -1;
+                        for (int q = 0; q < NP && r.pivot < 0; ++q)
+                            if (a.g.p[q]) r.pivot = 64 * q + __builtin_ctzll(a.g.p[q]);
+                        uint64_t u[4];
+                        axis_units(a.g, u);
+                        for (int w = 0; w < 4; ++w) {
+                            m3[w] |= u[w];  // union of the units: words 0..3
+                            for (uint64_t b = u[w]; b; b &= b - 1) pt[64 * w + __builtin_ctzll(b)] = (int16_t)j;
+                        }
+                        if (a.trunc_thresh) m3[3 * FFX_MAX_WORDS + (j >> 6)] |= 1ull << (j & 63);
+                        if (a.trunc_xw) m3[3 * FFX_MAX_WORDS + 4 + (j >> 6)] |= 1ull << (j & 63);
+                    }
+                    layer_masks.insert(layer_masks.end(), m3, m3 + kLM);
+                    ++li;
+                    continue;
+                }
                 for (uint32_t j = 0; j < layer_len[t]; ++j) {
                     const AStep& a = steps[t + j];
                     int site = 0, axis = 0;
                     single_site(a.g, site, axis);
-                    LayerGate<CP>& r = rec[layer_off[t] + j];
+                    LayerGate<CP>& r = rec[rec_off[t] + j];
                     r.cs = make_fac<CP>(a.cs);
                     r.sn = make_fac<CP>(a.sn);
                     r.site = site;
@@ -1129,7 +1204,40 @@ struct Engine {
                                    (hmix(0x9E3779B97F4A7C15ull * ++ctx->launch_seq) % (uint64_t)ctx->kernel_timing) == 0;
                 if (a.kind == 0) ctx->prof.run_launches += 1;
                 if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev], ctx->stream));
-                if (is_layer) {
+                if (is_layer && layer_kind[t] == 2) {
+                    constexpr int W = Alg::W;
+                    if (++ctx->layer_stamp >= (1u << 24)) {
+                        HIPCHK(hipMemsetAsync(ctx->claim, 0, ctx->hslots * sizeof(uint64_t), ctx->stream));
+                        ctx->layer_stamp = 1;
+                    }
+                    const size_t li = layer_index[t];
+                    DLayer<W, CP, Alg> L{};
+                    L.gates = (const DLayerGate<Alg, CP>*)((const char*)ctx->layer_buf + lrec2_off) + rec_off[t];
+                    L.pos = (const int16_t*)((const char*)ctx->layer_buf + lpos_off) + 256 * li;
+                    L.sd = (int32_t*)((char*)ctx->layer_buf + lsd_off) + layer_off[t];
+                    L.sd_stride = (uint32_t)lsd_stride;
+                    L.claim = ctx->claim;
+                    const uint64_t* lm = layer_masks.data() + li * kLM;
+                    for (int w = 0; w < W; ++w) L.umask[w] = lm[w];
+                    for (int u = 0; u < 4; ++u) {
+                        L.tmask[u] = lm[3 * FFX_MAX_WORDS + u];
+                        L.xmask[u] = lm[3 * FFX_MAX_WORDS + 4 + u];
+                    }
+                    L.mincoeff = mincoeff;
+                    L.nr = (int)kw;
+                    L.maxdeg = maxdeg;
+                    L.max_xw = max_xw;
+                    L.stamp = ctx->layer_stamp;
+                    hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K>), dim3(win_grid), dim3(kBlock), 0, ctx->stream,
+                                       store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L, ep, e);
+                    CHK(launch_check());
+                    if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev++ + 1], ctx->stream));
+                    std::swap(ctx->coef, ctx->coef2);
+                    step_of[e] = (int64_t)t;
+                    is_rot[e] = (int)kw;
+                    layer_of[e] = (int64_t)layer_off[t];
+                    ++e;
+                } else if (is_layer) {
                     if constexpr (Alg::kPauli) {
                         constexpr int W = Alg::W;
                         if (++ctx->layer_stamp >= (1u << 24)) {
@@ -1138,7 +1246,7 @@ struct Engine {
                         }
                         const size_t li = layer_index[t];
                         Layer<W, CP> L{};
-                        L.gates = (const LayerGate<CP>*)ctx->layer_buf + layer_off[t];
+                        L.gates = (const LayerGate<CP>*)ctx->layer_buf + rec_off[t];
                         L.pos = (const int16_t*)((const char*)ctx->layer_buf + lpos_off) + 256 * li;
                         L.sd = (int32_t*)((char*)ctx->layer_buf + lsd_off) + layer_off[t];
                         L.sd_stride = (uint32_t)lsd_stride;
@@ -1727,7 +1835,16 @@ int run_majorana(ffx_ctx* ctx, const ffx_majorana_gate* gates, int64_t n_gates, 
             prune_plan<Alg, CP, K>(sd, crots, *po);
             geo = prune_geo(*po);
         }
-        CHK(eng.run_async(sd, {}, mincoeff, maxdeg, -1, 0u, &crots, &geo, po ? po->log10_tau : 0.f));
+        // disjoint-support layers need every term's degree <= kLayerMaxA (as for Pauli layers)
+        bool layers = maxdegree <= kLayerMaxA;
+        for (int64_t i = 0; i < n_terms && layers; ++i) {
+            int d = 0;
+            for (int w = 0; w < W; ++w) d += __builtin_popcountll(m[i * W + w]);
+            layers = d <= kLayerMaxA;
+        }
+        if (const char* ev = getenv("FFX_LAYERS"))
+            if (std::strcmp(ev, "0") == 0) layers = false;
+        CHK(eng.run_async(sd, {}, mincoeff, maxdeg, -1, 0u, &crots, &geo, po ? po->log10_tau : 0.f, layers));
     } else
     for (int64_t i = n_gates - 1; i >= 0; --i) {
         const ffx_majorana_gate& g = gates[i];

@@ -147,47 +147,74 @@ def batch_kicked_ising(n_inst: int = 512, seed: int = 4242, l_lo: int = 6, l_spa
     return out
 
 
-def fermi_hubbard_terms(lx: int, ly: int, t: float = 1.0, u: float = 4.0, pbc: bool = False):
+def fermi_hubbard_terms(lx: int, ly: int, t: float = 1.0, u: float = 4.0, pbc: bool = False,
+                        order: str = "bonds"):
     """Fermi-Hubbard Hamiltonian -t sum_<ij>,s (a_is^dag a_js + h.c.) + U sum_i n_i,up n_i,dn
     on an lx x ly lattice as Majorana monomials [(indices, coefficient)], in the
     Trotter order used by fermi_hubbard (the constant U/4 per site is dropped).
     Site s = x + lx*y, mode 2s + spin, Majoranas 2*mode and 2*mode + 1 with
     a_j = (m_2j + i m_2j+1)/2, so n_j = (1 + i m_2j m_2j+1)/2 and
-    a_i^dag a_j + h.c. = (i/2)(m_2i m_2j+1 - m_2i+1 m_2j).  Bonds: horizontal
-    row by row, then vertical; with pbc also the wrap-around bonds."""
+    a_i^dag a_j + h.c. = (i/2)(m_2i m_2j+1 - m_2i+1 m_2j).
+
+    order "bonds": horizontal bonds row by row, then vertical ones (both spins,
+    both hopping monomials per bond), then per site the two degree-2 on-site
+    monomials and the degree-4 one.  order "layers": the bonds in four
+    matchings (horizontal bonds at even x, at odd x, vertical bonds at even y,
+    at odd y; a wrap-around bond that meets another bond of its matching gets a
+    matching of its own), then every degree-2 on-site monomial, then every
+    degree-4 one: each group's monomials have pairwise disjoint index sets.
+    With pbc also the wrap-around bonds (lattice sides above 2)."""
+    if order not in ("bonds", "layers"):
+        raise ValueError(f"unknown Trotter order {order!r}")
+
     def site(x, y):
         return x + lx * y
 
-    bonds = []
+    hb, vb = [], []  # (matching, s1, s2)
     for y in range(ly):
         for x in range(lx - 1 + (1 if pbc and lx > 2 else 0)):
-            bonds.append((site(x, y), site((x + 1) % lx, y)))
+            wrap = x == lx - 1
+            hb.append((2 if wrap and lx % 2 else x % 2, site(x, y), site((x + 1) % lx, y)))
     for y in range(ly - 1 + (1 if pbc and ly > 2 else 0)):
         for x in range(lx):
-            bonds.append((site(x, y), site(x, (y + 1) % ly)))
-    terms = []
-    for (s1, s2) in bonds:
+            wrap = y == ly - 1
+            vb.append((2 if wrap and ly % 2 else y % 2, site(x, y), site(x, (y + 1) % ly)))
+    if order == "bonds":
+        bonds = [(s1, s2) for _, s1, s2 in hb + vb]
+    else:
+        bonds = [(s1, s2) for grp in (hb, vb) for m in range(3) for mm, s1, s2 in grp if mm == m]
+
+    def hop(s1, s2):
         a, b = min(s1, s2), max(s1, s2)
+        out = []
         for spin in (0, 1):
             i, j = 2 * a + spin, 2 * b + spin
-            terms.append(([2 * i, 2 * j + 1], -0.5j * t))
-            terms.append(([2 * i + 1, 2 * j], 0.5j * t))
-    for s in range(lx * ly):
+            out.append(([2 * i, 2 * j + 1], -0.5j * t))
+            out.append(([2 * i + 1, 2 * j], 0.5j * t))
+        return out
+
+    def onsite(s):
         up, dn = 2 * s, 2 * s + 1
-        terms.append(([2 * up, 2 * up + 1], 0.25j * u))
-        terms.append(([2 * dn, 2 * dn + 1], 0.25j * u))
-        terms.append(([2 * up, 2 * up + 1, 2 * dn, 2 * dn + 1], -0.25 * u))
+        return [([2 * up, 2 * up + 1], 0.25j * u), ([2 * dn, 2 * dn + 1], 0.25j * u),
+                ([2 * up, 2 * up + 1, 2 * dn, 2 * dn + 1], -0.25 * u)]
+
+    terms = [m for (s1, s2) in bonds for m in hop(s1, s2)]
+    sites = [onsite(s) for s in range(lx * ly)]
+    if order == "bonds":
+        terms += [m for ms in sites for m in ms]
+    else:
+        terms += [m for ms in sites for m in ms[:2]] + [ms[2] for ms in sites]
     return terms
 
 
 def fermi_hubbard(lx: int = 6, ly: int = 6, t: float = 1.0, u: float = 4.0, dt: float = 0.05, steps: int = 20,
-                  pbc: bool = False):
+                  pbc: bool = False, order: str = "bonds"):
     """First-order Trotter circuit of fermi_hubbard_terms: per step every
     monomial c*M in order as MROT(M, c, 2 dt), i.e. theta = Im(c) 2dt for
     degree 2 (r = 1) and Re(c) 2dt for degree 4 (r = 0).  Returns (packed gate
     table, Majorana observable n_{0,up} = (1 + i m0 m1)/2 as (m, c))."""
     step = []
-    for idx, c in fermi_hubbard_terms(lx, ly, t, u, pbc):
+    for idx, c in fermi_hubbard_terms(lx, ly, t, u, pbc, order):
         r = (len(idx) >> 1) & 1
         step.append((idx, (c.imag if r else c.real) * 2 * dt, r))
     gates = pack_majorana(step * steps)

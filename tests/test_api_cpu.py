@@ -227,3 +227,29 @@ def test_fermi_hubbard_terms_vs_fermionic_hamiltonian():
     for idx, c in wl.fermi_hubbard_terms(lx, ly, t, u):
         M += c * D.majorana_matrix(idx, nm)
     assert np.allclose(H, M, atol=1e-13)
+
+
+@pytest.mark.parametrize("lx,ly,pbc", [(6, 6, False), (6, 6, True), (5, 5, True), (3, 4, False), (2, 2, False)])
+def test_fermi_hubbard_layered_order(lx, ly, pbc):
+    """The layered Trotter order holds the same monomials as the bond order, in
+    groups with pairwise disjoint index sets: four bond matchings (a fifth and
+    sixth for wrap-around bonds of odd sides), the degree-2 on-site monomials,
+    then the degree-4 ones."""
+    from fastfermion_amd import workloads as wl
+    a = wl.fermi_hubbard_terms(lx, ly, pbc=pbc)
+    b = wl.fermi_hubbard_terms(lx, ly, pbc=pbc, order="layers")
+    assert sorted((tuple(i), c) for i, c in a) == sorted((tuple(i), c) for i, c in b)
+    groups, used = 1, set()
+    for idx, _ in b:
+        if used & set(idx):
+            groups += 1
+            used = set()
+        used |= set(idx)
+    n_match = sum(1 for side in (lx, ly) for m in range(3)
+                  if m < 2 and side > 1 + m or (m == 2 and pbc and side > 2 and side % 2))
+    assert groups <= n_match + 2
+    # the last two groups: every on-site degree-2 monomial, then every degree-4 one
+    n = lx * ly
+    assert [len(i) for i, _ in b[-3 * n:]] == [2] * (2 * n) + [4] * n
+    with pytest.raises(ValueError):
+        wl.fermi_hubbard_terms(lx, ly, order="rows")

@@ -8,8 +8,8 @@
   C4  512 uneven kicked-Ising instances (L 6..12, so up to 144 qubits and 3 key
       words; depth 4..24) in one batched launch, FP64 and FP32 compensated: every
       instance's <O> and live term count against the oracle run per instance.
-  C5  the 6x6 Fermi-Hubbard Majorana circuit (20 Trotter steps, 6960 rotations),
-      bit-identical to the serial Majorana oracle (src/majorana_propagation.h:18-33);
+  C5  the 6x6 Fermi-Hubbard Majorana circuit (20 Trotter steps, 6960 rotations) in
+      both term orders, bit-identical to the serial Majorana oracle (src/majorana_propagation.h:18-33);
       and the learned pruner on the 30-layer kicked-Ising and the Fermi-Hubbard
       circuits within |d<O>| <= 1e-3 of the oracle.
 """
@@ -145,14 +145,18 @@ def test_c4_all_512_instances(ctx, c4, c4_oracle, prec, tol):
         assert len(bad_n) <= 2 and all(abs(int(nt[i]) - int(want_n[i])) <= 2 for i in bad_n)
 
 
-def test_c5_fermi_hubbard_6x6_bit_exact(ctx):
-    g, (m, c) = wl.fermi_hubbard(6, 6, steps=20)
+@pytest.mark.parametrize("order", ["bonds", "layers"])
+def test_c5_fermi_hubbard_6x6_bit_exact(ctx, order):
+    """Both Trotter term orders: bond order (rows of vertical bonds run as disjoint
+    layers) and layered order (six disjoint layers per step)."""
+    g, (m, c) = wl.fermi_hubbard(6, 6, steps=20, order=order)
     assert len(g) == 6960
     ref = pyffo.majorana_propagate(g, 3, m, c, maxdegree=8, mincoeff=1e-4)
     dev = ctx.majorana_propagate(g, 3, m, c, maxdegree=8, mincoeff=1e-4)
     p = ctx.profile()
-    print(f"C5 FH: {len(dev)} terms, {p['updates']} updates (oracle {ref.updates})", flush=True)
-    assert_same_terms(dev, ref, exact=True, what="Fermi-Hubbard 6x6")
+    print(f"C5 FH {order}: {len(dev)} terms, {p['updates']} updates (oracle {ref.updates}), "
+          f"{p['run_launches']} launches", flush=True)
+    assert_same_terms(dev, ref, exact=True, what=f"Fermi-Hubbard 6x6 {order}")
     assert p["updates"] == ref.updates
     occ = sum(1 << (2 * s) for s in range(36) if s % 2 == 0)
     occw = [occ & ((1 << 64) - 1), occ >> 64]
