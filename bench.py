@@ -53,6 +53,8 @@ def parse():
     ap.add_argument("--majorana", action="store_true",
                     help="time the Majorana path instead (for profiling): 6x6 Fermi-Hubbard, 20 Trotter steps, "
                          "maxdegree 8, mincoeff 1e-4, FP64 (no CPU leg, no side measurements)")
+    ap.add_argument("--fh-order", default="layers", choices=["layers", "bonds"],
+                    help="Trotter term order of the Fermi-Hubbard circuit of --majorana")
     return ap.parse_args()
 
 
@@ -163,38 +165,49 @@ def majorana_measurements(ctx, cpu_legs=True):
     """Majorana path at the Fermi-Hubbard size: 6x6 lattice, 72 modes (144 Majoranas,
     3 words), t=1, U=4, dt=0.05, 20 Trotter steps (6960 rotations), observable
     n_{0,up}, read out on the Neel-like Fock state with the up spins of the even
-    sites occupied; its roofline line (sampled launch times) and the oracle's
-    serial Majorana path (the reference has no threaded one) on the host."""
+    sites occupied, in both Trotter term orders (layered: six groups of
+    rotations on disjoint index sets per step, one launch each; bond order: rows
+    of vertical bonds as disjoint layers, windows elsewhere); each order's
+    roofline line (sampled launch times) and the oracle's serial Majorana path
+    (the reference has no threaded one) on the host, layered order."""
     from fastfermion_amd import workloads as wl
 
-    fh, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20)
     occ = sum(1 << (2 * s) for s in range(36) if s % 2 == 0)
     occw = [occ & ((1 << 64) - 1), occ >> 64]
-    run = lambda: ctx.majorana_propagate(fh, 3, fm, fc, maxdegree=8, mincoeff=1e-4, download=False,  # noqa: E731
-                                         occ=occw)
-    run()
-    reps, ks, kt, kl, upd, outs, efh = 5, 0.0, 0, 0, 0, 0, None
-    t = time.perf_counter()
-    for _ in range(reps):
-        _, efh = run()
-        p = ctx.profile()
-        ks += p["seconds_kernel"]
-        kt += p["timed_launches"]
-        kl += p["run_launches"]
-        upd += p["updates"]
-        outs += p["outs"]
-    dt = (time.perf_counter() - t) / reps
-    kavg = ks / kt if kt else 0.0
-    bpl = (8 * 3 + 8) * (upd + outs) / max(kl, 1)  # 24-byte keys + 8-byte coefficients, read and written
-    out = {"workload": "Fermi-Hubbard 6x6 (72 modes), t=1, U=4, dt=0.05, 20 Trotter steps, 6960 "
-                       "rotations, observable n_{0,up}, maxdegree 8, mincoeff 1e-4, FP64", "ms": 1e3 * dt,
-           "updates": upd // reps, "updates_per_s": upd / reps / dt, "peak_live_terms": int(p["peak_terms"]),
-           "expectation_neel": efh.real,
-           "roofline": {"bound": "hbm", "achieved": bpl / kavg / 1e9 if kavg else 0.0, "peak": HBM_PEAK / 1e9,
-                        "unit": "GB/s", "frac": bpl / kavg / HBM_PEAK if kavg else 0.0,
-                        "kernel": "k_win_async (windows of up to 4 commuting rotations) and k_rot_async",
-                        "kernel_ms_per_launch": 1e3 * kavg, "alg_bytes_per_launch": bpl,
-                        "launches_per_step": kl // reps, "timed_launches": kt}}
+
+    def measure(order):
+        fh, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20, order=order)
+        run = lambda: ctx.majorana_propagate(fh, 3, fm, fc, maxdegree=8, mincoeff=1e-4, download=False,  # noqa: E731
+                                             occ=occw)
+        run()
+        reps, ks, kt, kl, upd, outs, efh = 5, 0.0, 0, 0, 0, 0, None
+        t = time.perf_counter()
+        for _ in range(reps):
+            _, efh = run()
+            p = ctx.profile()
+            ks += p["seconds_kernel"]
+            kt += p["timed_launches"]
+            kl += p["run_launches"]
+            upd += p["updates"]
+            outs += p["outs"]
+        dt = (time.perf_counter() - t) / reps
+        kavg = ks / kt if kt else 0.0
+        bpl = (8 * 3 + 8) * (upd + outs) / max(kl, 1)  # 24-byte keys + 8-byte coefficients, read and written
+        return fh, fm, fc, {
+            "ms": 1e3 * dt, "updates": upd // reps, "updates_per_s": upd / reps / dt,
+            "peak_live_terms": int(p["peak_terms"]), "expectation_neel": efh.real,
+            "roofline": {"bound": "hbm", "achieved": bpl / kavg / 1e9 if kavg else 0.0, "peak": HBM_PEAK / 1e9,
+                         "unit": "GB/s", "frac": bpl / kavg / HBM_PEAK if kavg else 0.0,
+                         "kernel": "k_dlayer_async (disjoint layers), k_win_async (windows of up to 4 commuting "
+                                   "rotations) and k_rot_async, sampled launches",
+                         "kernel_ms_per_launch": 1e3 * kavg, "alg_bytes_per_launch": bpl,
+                         "launches_per_step": kl // reps, "timed_launches": kt}}
+
+    fh, fm, fc, out = measure("layers")
+    _, _, _, bond = measure("bonds")
+    out = {"workload": "Fermi-Hubbard 6x6 (72 modes), t=1, U=4, dt=0.05, 20 Trotter steps, 6960 rotations in the "
+                       "layered term order, observable n_{0,up}, maxdegree 8, mincoeff 1e-4, FP64", **out,
+           "bond_order": bond}
     if cpu_legs:
         sys.path.insert(0, os.path.join(ROOT, "oracle"))
         import pyffo
@@ -204,11 +217,11 @@ def majorana_measurements(ctx, cpu_legs=True):
             return r.seconds, r
         med, r, secs = timed_runs(one)
         out["cpu_baseline"] = {"value": r.updates / med, "unit": "term-gate updates/s", "cores": 1, "kind": "port",
-                               "sample": f"one full propagation ({r.updates} term-gate updates) by the serial "
-                                         f"restatement of the reference Majorana path (src/majorana_propagation.h:"
-                                         f"18-33, which has no threaded form), 1 warm-up + median of 3: "
-                                         f"{', '.join(f'{v:.3f}' for v in secs)} s",
-                               "gpu_speedup": (upd / reps / dt) / (r.updates / med)}
+                               "sample": f"one full propagation, layered order ({r.updates} term-gate updates) by "
+                                         f"the serial restatement of the reference Majorana path "
+                                         f"(src/majorana_propagation.h:18-33, which has no threaded form), "
+                                         f"1 warm-up + median of 3: {', '.join(f'{v:.3f}' for v in secs)} s",
+                               "gpu_speedup": out["updates_per_s"] / (r.updates / med)}
     return out
 
 
@@ -430,14 +443,14 @@ def learned_measurements(ctx, gates, W, x, z, c, plain_expect):
     out["kicked_ising_10x10_30_layers"] = {"reference": ref, "plain": plain, "learned": learned,
                                            **pareto(plain, learned)}
     # Fermi-Hubbard 6x6: model trained on logs of U = 3 and U = 5, applied at U = 4
-    fh, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20)
+    fh, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20, order="layers")
     frow, fcol = wl.hubbard_geometry(6, 6)
     frpl = len(fh) // 20
     occ = sum(1 << (2 * s) for s in range(36) if s % 2 == 0)
     occw = [occ & ((1 << 64) - 1), occ >> 64]
     fpo = prune_opts(frow, fcol, 0, rot_per_layer=frpl)
     t = time.perf_counter()
-    FF, FT = pruner.majorana_logs(ctx, [wl.fermi_hubbard(6, 6, u=u, steps=20)[0] for u in (3.0, 5.0)], 3, fm, fc, fpo,
+    FF, FT = pruner.majorana_logs(ctx, [wl.fermi_hubbard(6, 6, u=u, steps=20, order="layers")[0] for u in (3.0, 5.0)], 3, fm, fc, fpo,
                                   log_every=116, max_rows=1_500_000, maxdegree=8, mincoeff=1e-4, occ=occw)
     ctx.pruner_init(5)
     fh_hist = ctx.pruner_train(FF, FT, epochs=4, batch=4096, lr=1e-3, seed=2, under_weight=32.0)
@@ -496,7 +509,7 @@ def main():
     if a.majorana:
         # 72 modes (144 Majoranas, 3 words), t=1, U=4, dt=0.05, observable n_{0,up},
         # read out on the Fock state with the up spins of the even sites occupied
-        gates, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20)
+        gates, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20, order=a.fh_order)
         occ = sum(1 << (2 * s) for s in range(36) if s % 2 == 0)
         occ_w = [occ & ((1 << 64) - 1), occ >> 64]
         W, nq = 3, 72
@@ -577,8 +590,8 @@ def main():
                  else "synthetic (seeded circuit, single-Pauli observable)"),
         "config": {
             "workload": ("Fermi-Hubbard 6x6 (72 modes, 144 Majoranas), t=1, U=4, dt=0.05, 20 Trotter steps, "
-                         "observable n_{0,up}, maxdegree 8, mincoeff 1e-4 after every rotation, FP64"
-                         if a.majorana else
+                         f"{a.fh_order} term order, observable n_{{0,up}}, maxdegree 8, mincoeff 1e-4 after every "
+                         "rotation, FP64" if a.majorana else
                          f"kicked-Ising {L}x{L} lattice, {layers} layers, RZZ(-pi/2)+RX(theta_s in [0.2,0.6]), "
                          f"observable Z_{center}, maxdegree 8, mincoeff 1e-5, "
                          f"{'batched (reference default)' if a.batched else 'per-gate'} truncation"

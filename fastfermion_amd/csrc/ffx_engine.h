@@ -924,7 +924,9 @@ struct Engine {
     // Disjoint-support layer starting at rotation step t: consecutive rotations
     // whose axes have pairwise disjoint supports (Majorana strings of even
     // degree), at most kLayerMax; top-k and pruning points end it, compaction
-    // points inside move to its end; 0 when shorter than kLayerMin.
+    // points inside move to its end, and so does the start of a layer of
+    // single-qubit rotations (whose own kernel is faster); 0 when shorter than
+    // kLayerMin.
     static size_t plan_dlayer(const std::vector<AStep>& steps, size_t t) {
         uint64_t used[4] = {0, 0, 0, 0};
         size_t k = 0;
@@ -932,6 +934,7 @@ struct Engine {
             const AStep& a = steps[t + k];
             if (a.kind != 0) break;
             if (k > 0 && (steps[t + k - 1].topk_after || steps[t + k - 1].prune_after)) break;
+            if (Alg::kPauli && plan_layer(steps, t + k)) break;
             uint64_t u[4];
             axis_units(a.g, u);
             int cnt = 0;

@@ -190,13 +190,14 @@ def test_c5_learned_pruner_kicked_ising_30_layers(ctx):
 def test_c5_learned_pruner_fermi_hubbard(ctx):
     """The pruner trained on Fermi-Hubbard logs at U = 3 and U = 5 prunes the U = 4
     circuit every Trotter step; <n_0up> stays within 1e-3 of the serial oracle."""
-    fh, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20)
+    fh, (fm, fc) = wl.fermi_hubbard(6, 6, steps=20, order="layers")
     frow, fcol = wl.hubbard_geometry(6, 6)
     frpl = len(fh) // 20
     occ = sum(1 << (2 * s) for s in range(36) if s % 2 == 0)
     occw = [occ & ((1 << 64) - 1), occ >> 64]
     fpo = prune_opts(frow, fcol, 0, rot_per_layer=frpl)
-    F, T = pruner.majorana_logs(ctx, [wl.fermi_hubbard(6, 6, u=u, steps=20)[0] for u in (3.0, 5.0)], 3, fm, fc, fpo,
+    train = [wl.fermi_hubbard(6, 6, u=u, steps=20, order="layers")[0] for u in (3.0, 5.0)]
+    F, T = pruner.majorana_logs(ctx, train, 3, fm, fc, fpo,
                                 log_every=116, max_rows=1_500_000, maxdegree=8, mincoeff=1e-4, occ=occw)
     ctx.pruner_init(5)
     ctx.pruner_train(F, T, epochs=4, batch=4096, lr=1e-3, seed=2, under_weight=32.0)
