@@ -29,10 +29,7 @@ template <class Alg, class CP>
 struct DLayerGate {
     typename Alg::Gate g;
     typename CP::fac_t cs, sn;
-    int32_t pivot;   // private pivot bit (key bit index, word pivot / 64)
-    int32_t thresh;  // threshold truncation after this rotation
-    int32_t xw;      // X/Y-weight cap after this rotation
-    int32_t pad;
+    int32_t pivot;  // private pivot bit (key bit index, word pivot / 64)
 };
 
 template <int W, class CP, class Alg>
@@ -123,20 +120,42 @@ __device__ __forceinline__ void dlayer_canon(const uint64_t (&k)[Alg::NP], const
     }
 }
 
+// Active positions of an orbit (at most kLayerMaxA = 8, each < 256) packed in
+// layer order, 8 bits each.
+__device__ __forceinline__ uint64_t pack_positions(const uint64_t (&pm)[4]) {
+    uint64_t ap = 0;
+    int d = 0;
+#pragma unroll
+    for (int u = 0; u < 4; ++u) {
+        uint64_t b = pm[u];
+        while (b) {
+            ap |= (uint64_t)(64 * u + __builtin_ctzll(b)) << (8 * d);
+            b &= b - 1;
+            ++d;
+        }
+    }
+    return ap;
+}
+
+// d-th packed position, nr past the last of the a positions.
+__device__ __forceinline__ int apos(uint64_t ap, int d, int a, int nr) {
+    return d < a ? (int)((ap >> (8 * d)) & 0xFFu) : nr;
+}
+
 // Member T of the orbit of canonical key ck: the axes of the active positions
 // whose bit d (d-th active position in layer order) is set in T.
 template <class Alg, class CP>
-__device__ __forceinline__ void dlayer_member(const uint64_t (&ck)[Alg::NP], const uint64_t (&pm)[4], int a, int T,
+__device__ __forceinline__ void dlayer_member(const uint64_t (&ck)[Alg::NP], uint64_t ap, int a, int T,
                                               const DLayerGate<Alg, CP>* sg, uint64_t (&key)[Alg::NP]) {
     constexpr int NP = Alg::NP;
 #pragma unroll
     for (int q = 0; q < NP; ++q) key[q] = ck[q];
-    uint64_t pw[4] = {pm[0], pm[1], pm[2], pm[3]};
     for (int d = 0; d < a; ++d) {
-        const int pq = pop_pos(pw, 0);
-        if ((T >> d) & 1)
+        if ((T >> d) & 1) {
+            const int pq = (int)((ap >> (8 * d)) & 0xFFu);
 #pragma unroll
             for (int q = 0; q < NP; ++q) key[q] ^= sg[pq].g.p[q];
+        }
     }
 }
 
@@ -153,10 +172,11 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
     __shared__ int32_t ssd[kLayerMax];
     __shared__ uint32_t s_cnt, s_base;
     // leaders of the block iteration, by number of active axes (1..8): their keys and
-    // active position masks, so that any wave of the block can replay any leader's orbit
+    // packed active positions, so that any wave of the block can replay any leader's orbit
     __shared__ uint64_t slk[kBlock][NP];
-    __shared__ uint64_t spm[kBlock][4];
-    __shared__ uint16_t slist[kLayerMaxA + 1][kBlock];
+    __shared__ uint64_t sap[kBlock];
+    __shared__ uint8_t slist[kLayerMaxA + 1][kBlock];
+    static_assert(kBlock <= 256, "leader lists hold 8-bit thread indices");
     __shared__ uint32_t scount[kLayerMaxA + 1];
     uint32_t n, dead;
     if (!step_enter(ep, e, n, dead)) return;
@@ -286,9 +306,8 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                 if (leader) {
 #pragma unroll
                     for (int q = 0; q < NP; ++q) slk[threadIdx.x][q] = ck[q];
-#pragma unroll
-                    for (int u = 0; u < 4; ++u) spm[threadIdx.x][u] = pm[u];
-                    slist[a][atomicAdd(&scount[a], 1u)] = (uint16_t)threadIdx.x;
+                    sap[threadIdx.x] = pack_positions(pm);
+                    slist[a][atomicAdd(&scount[a], 1u)] = (uint8_t)threadIdx.x;
                 }
             }
         }
@@ -305,12 +324,12 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                 const int kth = pass * per + g;
                 const bool on = kth < nlead;
                 const int src = on ? (int)slist[A][kth] : 0;
-                uint64_t ck[NP], pm[4], key[NP];
+                uint64_t ck[NP], key[NP];
 #pragma unroll
                 for (int q = 0; q < NP; ++q) ck[q] = on ? slk[src][q] : 0ull;
-#pragma unroll
-                for (int u = 0; u < 4; ++u) pm[u] = on ? spm[src][u] : 0ull;
-                dlayer_member<Alg, CP>(ck, pm, on ? A : 0, T, sg, key);
+                const uint64_t ap = on ? sap[src] : 0ull;
+                const int Aon = on ? A : 0;
+                dlayer_member<Alg, CP>(ck, ap, Aon, T, sg, key);
                 const bool recv_ok = Alg::degree(key) <= L.maxdeg;
                 bool exist = false, alive = false;
                 uint32_t slot = kNone;
@@ -341,10 +360,9 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                         atomicAdd(&ssd[px], -1);
                     }
                 };
-                uint64_t pw[4] = {pm[0], pm[1], pm[2], pm[3]};
-                trunc1(0, peek_pos(pw, L.nr));
+                trunc1(0, apos(ap, 0, Aon, L.nr));
                 for (int d = 0; d < A; ++d) {
-                    const int pq = pop_pos(pw, L.nr);
+                    const int pq = apos(ap, d, Aon, L.nr);
                     const DLayerGate<Alg, CP>& lg = sg[on ? pq : 0];
                     val_t vu[K];
 #pragma unroll
@@ -371,7 +389,7 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                             alive = true;
                         }
                         if (alive != before) atomicAdd(&ssd[pq], alive ? 1 : -1);
-                        trunc1(pq, peek_pos(pw, L.nr));
+                        trunc1(pq, apos(ap, d + 1, Aon, L.nr));
                     }
                 }
                 if (on) {
@@ -397,11 +415,10 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
         for (int jb = ((wave - poff) % nwave + nwave) % nwave; jb < n78; jb += nwave) {
             const int src = jb < n7 ? (int)slist[7][jb] : (int)slist[8][jb - n7];
             const int a = jb < n7 ? 7 : 8;
-            uint64_t ck[NP], pm[4];
+            uint64_t ck[NP];
 #pragma unroll
             for (int q = 0; q < NP; ++q) ck[q] = slk[src][q];
-#pragma unroll
-            for (int u = 0; u < 4; ++u) pm[u] = spm[src][u];
+            const uint64_t ap = sap[src];
             const int nm = 1 << a;
             // members T = lane + 64 m
             uint64_t key[4][NP];
@@ -415,7 +432,7 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                 slot[m] = kNone;
 #pragma unroll
                 for (int kk = 0; kk < K; ++kk) v[m][kk] = CP::from_double(0.0);
-                dlayer_member<Alg, CP>(ck, pm, T < nm ? a : 0, T, sg, key[m]);
+                dlayer_member<Alg, CP>(ck, ap, T < nm ? a : 0, T, sg, key[m]);
                 recv_ok[m] = Alg::degree(key[m]) <= L.maxdeg;
                 if (T < nm) {
                     slot[m] = probe<NP>(st.key, cap, st.index, st.hmask, key[m], n, &st.ctrl->index_full);
@@ -459,10 +476,9 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                     if (kx) atomicAdd(&ssd[px], -kx);
                 }
             };
-            uint64_t pw[4] = {pm[0], pm[1], pm[2], pm[3]};
-            truncate_range(0, peek_pos(pw, L.nr));
+            truncate_range(0, apos(ap, 0, a, L.nr));
             for (int d = 0; d < a; ++d) {
-                const int pq = pop_pos(pw, L.nr);
+                const int pq = apos(ap, d, a, L.nr);
                 const DLayerGate<Alg, CP>& lg = sg[pq];
                 const int bitd = 1 << d;
                 int before = 0, after = 0;
@@ -513,7 +529,7 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                 int delta = after - before;
                 for (int o = 32; o > 0; o >>= 1) delta += __shfl_xor(delta, o, 64);
                 if (lane == 0 && delta) atomicAdd(&ssd[pq], delta);
-                truncate_range(pq, peek_pos(pw, L.nr));
+                truncate_range(pq, apos(ap, d + 1, a, L.nr));
             }
             // write back existing members, stage fresh live ones
 #pragma unroll

@@ -1093,8 +1093,6 @@ struct Engine {
                         r.g = a.g;
                         r.cs = make_fac<CP>(a.cs);
                         r.sn = make_fac<CP>(a.sn);
-                        r.thresh = a.trunc_thresh;
-                        r.xw = a.trunc_xw;
                         r.pivot = -1;
                         for (int q = 0; q < NP && r.pivot < 0; ++q)
                             if (a.g.p[q]) r.pivot = 64 * q + __builtin_ctzll(a.g.p[q]);
