@@ -18,6 +18,15 @@
 // order with the one-rotation kernel's operations (per-member degree filter and
 // sign), truncation points between stages, staged appends.
 //
+// General layers: at most kLayerMaxA (8) consecutive rotations that commute
+// pairwise and have linearly independent axes, supports overlapping.  The set
+// A(q) is again an orbit invariant (commutation is a bicharacter, so flipping q
+// by P_j leaves q's commutation with every P_k unchanged when P_j and P_k
+// commute), |A| <= 8 because the layer holds at most 8 rotations, and the 2^|A|
+// members are distinct because the axes are independent; with no private pivot
+// bits, the canonical member is the orbit's smallest key, found by a Gray-code
+// walk over the orbit.
+//
 // Requires every live term's degree <= kLayerMaxA (host check as for layers).
 #pragma once
 
@@ -46,6 +55,7 @@ struct DLayer {
     int maxdeg;
     int max_xw;
     uint32_t stamp;  // this launch's stamp (24 bits, nonzero)
+    int general;     // 1: at most kLayerMaxA commuting rotations with overlapping supports
 };
 
 // Units of a key (qubits in the Pauli support, Majorana indices), word w.
@@ -91,6 +101,42 @@ __device__ __forceinline__ void dlayer_active(const uint64_t (&k)[Alg::NP], cons
                 ++na;
             }
         }
+    }
+}
+
+// Positions of a general layer's rotations (nr <= kLayerMaxA) that anticommute with k.
+template <class Alg, class CP>
+__device__ __forceinline__ void dlayer_active_all(const uint64_t (&k)[Alg::NP], const DLayerGate<Alg, CP>* sg,
+                                                  int nr, uint64_t (&pm)[4], int& na) {
+    pm[0] = pm[1] = pm[2] = pm[3] = 0;
+    na = 0;
+    for (int j = 0; j < nr; ++j)
+        if (Alg::anticommutes(k, sg[j].g)) {
+            pm[0] |= 1ull << j;
+            ++na;
+        }
+}
+
+// Canonical member of a general layer's orbit: the smallest key (last word most
+// significant) of k ^ span{active axes}, visited in Gray-code order.
+template <class Alg, class CP>
+__device__ __forceinline__ void dlayer_canon_min(const uint64_t (&k)[Alg::NP], uint64_t ap, int a,
+                                                 const DLayerGate<Alg, CP>* sg, uint64_t (&ck)[Alg::NP]) {
+    constexpr int NP = Alg::NP;
+    uint64_t cur[NP];
+#pragma unroll
+    for (int q = 0; q < NP; ++q) ck[q] = cur[q] = k[q];
+    for (int i = 1; i < (1 << a); ++i) {
+        const int pq = (int)((ap >> (8 * __builtin_ctz(i))) & 0xFFu);
+#pragma unroll
+        for (int q = 0; q < NP; ++q) cur[q] ^= sg[pq].g.p[q];
+        int c = 0;  // sign of cur - ck, decided by the most significant differing word
+#pragma unroll
+        for (int q = 0; q < NP; ++q)
+            if (cur[q] != ck[q]) c = cur[q] < ck[q] ? -1 : 1;
+        if (c < 0)
+#pragma unroll
+            for (int q = 0; q < NP; ++q) ck[q] = cur[q];
     }
 }
 
@@ -228,7 +274,10 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
             uint64_t k[NP], pm[4];
             key_load<NP>(st.key, i, k);
             int a = 0;
-            dlayer_active<Alg, CP>(k, L.umask, spos, sg, pm, a);
+            if (L.general)
+                dlayer_active_all<Alg, CP>(k, sg, L.nr, pm, a);
+            else
+                dlayer_active<Alg, CP>(k, L.umask, spos, sg, pm, a);
             if (a == 0) {
                 // untouched by every rotation of the layer; the truncation points inside
                 // the layer still apply (to an unchanged value: once is enough)
@@ -267,7 +316,10 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                     CP::store(cout + (size_t)kk * CP::NS * cap, cap, i, CP::load(cin + (size_t)kk * CP::NS * cap, cap, i));
             } else {
                 uint64_t ck[NP];
-                dlayer_canon<Alg, CP>(k, pm, sg, ck);
+                if (L.general)
+                    dlayer_canon_min<Alg, CP>(k, pack_positions(pm), a, sg, ck);
+                else
+                    dlayer_canon<Alg, CP>(k, pm, sg, ck);
                 const uint64_t h = hmix(lin_hash<NP>(ck));
                 const uint64_t tag8 = (h >> 56) & 0xFFull;
                 const uint64_t mine = ((uint64_t)L.stamp << 40) | (tag8 << 32) | (uint64_t)i;
@@ -294,8 +346,13 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
                         uint64_t kj[NP], pmj[4], ckj[NP];
                         int aj = 0;
                         key_load<NP>(st.key, j, kj);
-                        dlayer_active<Alg, CP>(kj, L.umask, spos, sg, pmj, aj);
-                        dlayer_canon<Alg, CP>(kj, pmj, sg, ckj);
+                        if (L.general) {
+                            dlayer_active_all<Alg, CP>(kj, sg, L.nr, pmj, aj);
+                            dlayer_canon_min<Alg, CP>(kj, pack_positions(pmj), aj, sg, ckj);
+                        } else {
+                            dlayer_active<Alg, CP>(kj, L.umask, spos, sg, pmj, aj);
+                            dlayer_canon<Alg, CP>(kj, pmj, sg, ckj);
+                        }
                         bool eq = true;
 #pragma unroll
                         for (int q = 0; q < NP; ++q) eq = eq && ckj[q] == ck[q];

@@ -950,6 +950,42 @@ struct Engine {
         return k >= (size_t)kLayerMin ? k : 0;
     }
 
+    // General layer starting at rotation step t: up to kLayerMaxA consecutive
+    // rotations that commute pairwise with linearly independent axes (supports may
+    // overlap); ends like a disjoint layer; 0 when shorter than kLayerMin.
+    static size_t plan_glayer(const std::vector<AStep>& steps, size_t t) {
+        uint64_t basis[kLayerMaxA][NP];
+        int pivot[kLayerMaxA];
+        int nb = 0;
+        auto add = [&](const uint64_t* p) {
+            uint64_t v[NP];
+            for (int q = 0; q < NP; ++q) v[q] = p[q];
+            for (int b = 0; b < nb; ++b)
+                if ((v[pivot[b] >> 6] >> (pivot[b] & 63)) & 1)
+                    for (int q = 0; q < NP; ++q) v[q] ^= basis[b][q];
+            for (int q = 0; q < NP; ++q)
+                if (v[q]) {
+                    pivot[nb] = 64 * q + __builtin_ctzll(v[q]);
+                    for (int r = 0; r < NP; ++r) basis[nb][r] = v[r];
+                    ++nb;
+                    return true;
+                }
+            return false;
+        };
+        size_t k = 0;
+        while (t + k < steps.size() && k < (size_t)kLayerMaxA) {
+            const AStep& a = steps[t + k];
+            if (a.kind != 0) break;
+            if (k > 0 && (steps[t + k - 1].topk_after || steps[t + k - 1].prune_after)) break;
+            if (k > 0 && Alg::kPauli && plan_layer(steps, t + k)) break;
+            bool ok = true;
+            for (size_t j = 0; j < k && ok; ++j) ok = axes_commute(steps[t + j].g, a.g);
+            if (!ok || !add(a.g.p)) break;
+            ++k;
+        }
+        return k >= (size_t)kLayerMin ? k : 0;
+    }
+
     TopK topk_scratch() const {
         TopK tk;
         tk.hist = ctx->topk_buf;
@@ -1031,7 +1067,7 @@ struct Engine {
         // supports, ffx_dlayer.h); layer_off numbers their rotations for the counters,
         // rec_off their records (each kind in its own array)
         std::vector<uint32_t> layer_len(steps.size(), 0), layer_off(steps.size(), 0), rec_off(steps.size(), 0);
-        std::vector<uint8_t> layer_kind(steps.size(), 0);
+        std::vector<uint8_t> layer_kind(steps.size(), 0), layer_general(steps.size(), 0);
         size_t layer_bytes = 0, n_layer_rot = 0, n_rec1 = 0, n_rec2 = 0;
         if (layers) {
             const bool dlayers = getenv("FFX_NO_DLAYERS") == nullptr;  // kind 2 off: a cross-check
@@ -1042,7 +1078,13 @@ struct Engine {
                     k = plan_layer(steps, t);
                     kind = k ? 1 : 0;
                 }
-                if (!k && dlayers && (k = plan_dlayer(steps, t)) != 0) kind = 2;
+                if (!k && dlayers) {
+                    // the longer of a disjoint and a general layer (the disjoint one on ties)
+                    const size_t kd = plan_dlayer(steps, t), kg = plan_glayer(steps, t);
+                    k = std::max(kd, kg);
+                    kind = k ? 2 : 0;
+                    layer_general[t] = kg > kd ? 1 : 0;
+                }
                 if (k) {
                     layer_len[t] = (uint32_t)k;
                     layer_off[t] = (uint32_t)n_layer_rot;
@@ -1229,6 +1271,7 @@ struct Engine {
                     L.maxdeg = maxdeg;
                     L.max_xw = max_xw;
                     L.stamp = ctx->layer_stamp;
+                    L.general = layer_general[t];
                     hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K>), dim3(win_grid), dim3(kBlock), 0, ctx->stream,
                                        store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L, ep, e);
                     CHK(launch_check());

@@ -199,3 +199,77 @@ def test_disjoint_layers_reduced_precision_identical(ctx, monkeypatch, prec):
                                  lambda: ctx.majorana_propagate(g, 1, m, c, maxdegree=8, mincoeff=1e-4,
                                                                 precision=prec))
     assert_same_terms(dev, off, exact=True, what=f"FH prec={prec}")
+
+
+def pauli_commuting_circuit(rng, nq, blocks):
+    """Runs of 3..12 commuting rotations with overlapping supports (Z-type or X-type
+    strings of weight 2..3 on nearby qubits, some closing cycles, i.e. linearly
+    dependent on earlier axes of the run), with single-qubit and random two-qubit
+    rotations between the runs."""
+    spec = []
+    for _ in range(blocks):
+        op = str(rng.choice(["Z", "X"]))
+        base = int(rng.integers(0, nq - 12))
+        for _ in range(int(rng.integers(3, 13))):
+            w = int(rng.integers(2, 4))
+            qs = tuple(sorted((base + rng.choice(12, w, replace=False)).tolist()))
+            spec.append((op * w, qs, float(rng.uniform(-1.2, 1.2))))
+        for _ in range(int(rng.integers(1, 4))):
+            if rng.random() < 0.5:
+                spec.append((str(rng.choice(list("XYZ"))), (int(rng.integers(nq)),), float(rng.uniform(-1.2, 1.2))))
+            else:
+                a, b = sorted(rng.choice(nq, 2, replace=False).tolist())
+                spec.append(("".join(rng.choice(list("XYZ"), 2)), (a, b), float(rng.uniform(-1.2, 1.2))))
+    return wl._pack(spec)
+
+
+@pytest.mark.parametrize("nq,seed,blocks", [(40, 0, 12), (100, 1, 14), (200, 2, 14)])
+@pytest.mark.parametrize("batched", [True, False])
+def test_pauli_general_layers_bit_exact(ctx, monkeypatch, nq, seed, blocks, batched):
+    """General layers (up to eight commuting rotations on overlapping supports, one
+    launch) against the oracle, and against the run without them."""
+    rng = np.random.default_rng(40 + seed)
+    W = _abi.words_for_bits(nq)
+    g = pauli_commuting_circuit(rng, nq, blocks)
+    x, z, c = pauli_observable(rng, nq, W, 20)
+    kw = dict(maxdegree=8, mincoeff=1e-6, batched=batched)
+    ref = pyffo.pauli_propagate(g, W, x, z, c, **kw)
+    off, dev, pr, pg = _launches(ctx, monkeypatch, lambda: ctx.pauli_propagate(g, W, x, z, c, **kw))
+    print(f"Pauli general nq={nq} batched={batched}: {len(dev)} terms, launches {pr['run_launches']} -> "
+          f"{pg['run_launches']}", flush=True)
+    assert_same_terms(dev, ref, exact=True, what=f"Pauli general nq={nq} batched={batched}")
+    assert_same_terms(off, ref, exact=True, what="without general layers")
+    assert pg["updates"] == ref.updates
+    assert pg["run_launches"] < pr["run_launches"]
+
+
+@pytest.mark.parametrize("seed", range(3))
+def test_majorana_general_layers_bit_exact(ctx, monkeypatch, seed):
+    """Degree-4 strings made of two index pairs {2i, 2i+1}: any two overlap in 0 or 2
+    indices, so runs of them commute with overlapping supports (three strings on
+    the pairs of a triangle are linearly dependent and end a run)."""
+    rng = np.random.default_rng(60 + seed)
+    n, W = 120, 2
+    spec = []
+    for _ in range(12):
+        base = int(rng.integers(0, n // 2 - 6))
+        for _ in range(int(rng.integers(3, 12))):
+            p1, p2 = sorted((base + rng.choice(6, 2, replace=False)).tolist())
+            spec.append(([2 * p1, 2 * p1 + 1, 2 * p2, 2 * p2 + 1], float(rng.uniform(-1.0, 1.0)), 0))
+        for _ in range(int(rng.integers(1, 3))):
+            d = int(rng.integers(1, 5))
+            idx = sorted(rng.choice(n, d, replace=False).tolist())
+            spec.append((idx, float(rng.uniform(-1.0, 1.0)), (d >> 1) & 1))
+    g = wl.pack_majorana(spec)
+    m = np.zeros((6, W), np.uint64)
+    for i in range(6):
+        for a in rng.choice(n, 2 * int(rng.integers(1, 3)), replace=False):
+            m[i, int(a) >> 6] |= np.uint64(1) << np.uint64(int(a) & 63)
+    c = rng.normal(size=6) + 1j * rng.normal(size=6)
+    ref = pyffo.majorana_propagate(g, W, m, c, maxdegree=8, mincoeff=1e-5)
+    off, dev, pr, pg = _launches(ctx, monkeypatch,
+                                 lambda: ctx.majorana_propagate(g, W, m, c, maxdegree=8, mincoeff=1e-5))
+    assert_same_terms(dev, ref, exact=True, what=f"Majorana general seed={seed}")
+    assert_same_terms(off, ref, exact=True, what="without general layers")
+    assert pg["updates"] == ref.updates
+    assert pg["run_launches"] < pr["run_launches"]
