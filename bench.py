@@ -40,8 +40,9 @@ def parse():
     ap.add_argument("--precision", default="fp64", choices=["fp64", "fp32", "fp32c"])
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline leg")
     ap.add_argument("--cpu-threads", type=int, default=0)
-    ap.add_argument("--timing-every", type=int, default=16,
-                    help="time every N-th propagation launch with HIP events (0: off)")
+    ap.add_argument("--timing-every", type=int, default=1,
+                    help="after the timed steps, one more propagation with every N-th propagation launch timed "
+                         "by HIP events on the engine's stream, for the roofline's kernel average (0: off)")
     ap.add_argument("--no-extras", action="store_true",
                     help="skip the precision, pruned and batched side measurements")
     ap.add_argument("--pruned", action="store_true",
@@ -168,7 +169,7 @@ def majorana_measurements(ctx, cpu_legs=True):
     sites occupied, in both Trotter term orders (layered: six groups of
     rotations on disjoint index sets per step, one launch each; bond order: rows
     of vertical bonds as disjoint layers, windows elsewhere); each order's
-    roofline line (sampled launch times) and the oracle's serial Majorana path
+    roofline line (launch times of one extra, event-timed run) and the oracle's serial Majorana path
     (the reference has no threaded one) on the host, layered order."""
     from fastfermion_amd import workloads as wl
 
@@ -180,17 +181,21 @@ def majorana_measurements(ctx, cpu_legs=True):
         run = lambda: ctx.majorana_propagate(fh, 3, fm, fc, maxdegree=8, mincoeff=1e-4, download=False,  # noqa: E731
                                              occ=occw)
         run()
-        reps, ks, kt, kl, upd, outs, efh = 5, 0.0, 0, 0, 0, 0, None
+        reps, kl, upd, outs, efh = 5, 0, 0, 0, None
         t = time.perf_counter()
         for _ in range(reps):
             _, efh = run()
             p = ctx.profile()
-            ks += p["seconds_kernel"]
-            kt += p["timed_launches"]
             kl += p["run_launches"]
             upd += p["updates"]
             outs += p["outs"]
         dt = (time.perf_counter() - t) / reps
+        # kernel durations from one more run with every propagation launch event-timed
+        ctx.set_kernel_timing(1)
+        run()
+        pk = ctx.profile()
+        ctx.set_kernel_timing(0)
+        ks, kt = pk["seconds_kernel"], pk["timed_launches"]
         kavg = ks / kt if kt else 0.0
         bpl = (8 * 3 + 8) * (upd + outs) / max(kl, 1)  # 24-byte keys + 8-byte coefficients, read and written
         return fh, fm, fc, {
@@ -199,7 +204,7 @@ def majorana_measurements(ctx, cpu_legs=True):
             "roofline": {"bound": "hbm", "achieved": bpl / kavg / 1e9 if kavg else 0.0, "peak": HBM_PEAK / 1e9,
                          "unit": "GB/s", "frac": bpl / kavg / HBM_PEAK if kavg else 0.0,
                          "kernel": "k_dlayer_async (disjoint layers), k_win_async (windows of up to 4 commuting "
-                                   "rotations) and k_rot_async, sampled launches",
+                                   "rotations) and k_rot_async; every launch of one extra run event-timed",
                          "kernel_ms_per_launch": 1e3 * kavg, "alg_bytes_per_launch": bpl,
                          "launches_per_step": kl // reps, "timed_launches": kt}}
 
@@ -504,7 +509,7 @@ def main():
         print(json.dumps({"learned_pruner": learned_measurements(ctx, gates, W, x, z, c, None)}), flush=True)
         ctx.close()
         return
-    ctx.set_kernel_timing(a.timing_every)
+    ctx.set_kernel_timing(0)
     step = lambda: ctx.pauli_propagate(gates, W, x, z, c, download=False, want_expect=True, **kw)  # noqa: E731
     if a.majorana:
         # 72 modes (144 Majoranas, 3 words), t=1, U=4, dt=0.05, observable n_{0,up},
@@ -527,8 +532,6 @@ def main():
     for _ in range(a.warmup):
         step()
     updates = outs = 0
-    kernel_s = 0.0
-    timed_launches = 0
     kernel_launches = 0
     enqueue_s = 0.0
     expect = None
@@ -538,33 +541,48 @@ def main():
         p = ctx.profile()
         updates += p["updates"]
         outs += p["outs"]
-        kernel_s += p["seconds_kernel"]
-        timed_launches += p["timed_launches"]
         enqueue_s += p["seconds_enqueue"]
         kernel_launches += p["run_launches"]
     t1 = time.perf_counter()
     wall = t1 - t0
     peak_terms = p["peak_terms"]
     value = updates / wall
+    # kernel durations: one more (untimed) propagation with HIP events around the
+    # propagation launches, so the events do not slow the timed steps
+    kernel_s, timed_launches, timed_run_launches = 0.0, 0, 0
+    if a.timing_every > 0:
+        ctx.set_kernel_timing(a.timing_every)
+        step()
+        p = ctx.profile()
+        kernel_s, timed_launches, timed_run_launches = p["seconds_kernel"], p["timed_launches"], p["run_launches"]
+        ctx.set_kernel_timing(0)
 
     # algorithmic bytes of the propagation kernel: (key + coefficient bytes) per
     # live term read and per live term written, per rotation
     Kb = (8 if a.majorana else 16) * W
     Cb = {_abi.FP64: 8, _abi.FP32: 4, _abi.FP32C: 8}[prec]
     alg_bytes = (Kb + Cb) * (updates + outs)
-    # average duration of the sampled launches; algorithmic bytes per launch
+    # average duration of the event-timed launches; algorithmic bytes per launch
     kernel_avg = kernel_s / timed_launches if timed_launches else 0.0
     bytes_per_launch = alg_bytes / max(kernel_launches, 1)
     achieved = bytes_per_launch / kernel_avg if kernel_avg > 0 else 0.0
 
-    # HBM bytes per launch of the same kernel and workload, from the two rocprofv3
-    # --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled against a plain copy kernel
-    traffic = None
+    # HBM bytes per launch of the same kernels and workload, from the two rocprofv3
+    # --pmc runs (FETCH_SIZE, WRITE_SIZE) scaled against a plain copy kernel; used
+    # only while the recorded launch and update counts match this run's (a change of
+    # kernels or schedule makes the record stale: then null, with the reason)
+    traffic, traffic_note = None, "no counter record for this configuration"
     tf = os.path.join(ROOT, "profiles", "propagation_kicked_ising_10x10_fp64_traffic.json")
     if (L, layers, a.batched, a.precision, a.pruned, a.majorana) == (10, 20, 1, "fp64", False, False) \
             and os.path.exists(tf):
         with open(tf) as fh:
-            traffic = json.load(fh)["bytes_per_launch"]
+            rec = json.load(fh)
+        now = (kernel_launches // a.steps, updates // a.steps)
+        if (rec.get("launches_per_step"), rec.get("updates_per_step")) == now:
+            traffic, traffic_note = rec["bytes_per_launch"], f"from {os.path.relpath(tf, ROOT)}"
+        else:
+            traffic_note = (f"stale counter record ({rec.get('launches_per_step')} launches, "
+                            f"{rec.get('updates_per_step')} updates per step; this run {now[0]}, {now[1]})")
 
     cpu = None
     oracle_expect = None
@@ -611,10 +629,12 @@ def main():
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK,
             "traffic": traffic,
-            "kernel": ("propagation launches: k_win_async (a window of up to 4 commuting rotations) and "
-                       "k_rot_async (one rotation)" if a.majorana else
-                       "propagation launches: k_win_async (a window of up to 4 commuting rotations) and "
-                       "k_layer_async (a layer of single-qubit rotations)"),
+            "traffic_source": traffic_note,
+            "kernel": ("propagation launches: k_dlayer_async (one launch per group of rotations on disjoint "
+                       "index sets)" if a.majorana else
+                       "propagation launches: k_dlayer_async (general layers of up to 8 commuting RZZ, disjoint "
+                       "layers of up to 10 vertical RZZ), k_layer_async (each RX layer of 100 single-qubit "
+                       "rotations), k_win_async (leftover windows of up to 4 commuting rotations)"),
             "kernel_ms_per_launch": 1e3 * kernel_avg,
             "alg_bytes_per_launch": bytes_per_launch,
             "launches_per_step": kernel_launches // a.steps,

@@ -34,6 +34,10 @@
 
 namespace ffx {
 
+// appends staged in LDS per block iteration (more go straight to HBM); 256 keeps
+// the kernel at 4 blocks of 256 threads per CU for keys of up to 4 words
+constexpr int kDLayerStage = 256;
+
 template <class Alg, class CP>
 struct DLayerGate {
     typename Alg::Gate g;
@@ -213,8 +217,8 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
     using val_t = typename CP::val_t;
     __shared__ int16_t spos[256];
     __shared__ DLayerGate<Alg, CP> sg[kLayerMax];
-    __shared__ uint64_t skey[kLayerStage][NP];
-    __shared__ val_t sval[kLayerStage][K];
+    __shared__ uint64_t skey[kDLayerStage][NP];
+    __shared__ val_t sval[kDLayerStage][K];
     __shared__ int32_t ssd[kLayerMax];
     __shared__ uint32_t s_cnt, s_base;
     // leaders of the block iteration, by number of active axes (1..8): their keys and
@@ -249,7 +253,7 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
     // stage a fresh live member for appending (one global reservation per block iteration)
     auto stage_append = [&](const uint64_t (&key)[NP], const val_t (&v)[K]) {
         const uint32_t r = atomicAdd(&s_cnt, 1u);
-        if (r < (uint32_t)kLayerStage) {
+        if (r < (uint32_t)kDLayerStage) {
 #pragma unroll
             for (int q = 0; q < NP; ++q) skey[r][q] = key[q];
 #pragma unroll
@@ -612,7 +616,7 @@ __global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const 
         }
         __syncthreads();
         // ---- phase 3: one reservation per block for the staged appends ----
-        const uint32_t staged = min(s_cnt, (uint32_t)kLayerStage);
+        const uint32_t staged = min(s_cnt, (uint32_t)kDLayerStage);
         if (threadIdx.x == 0 && staged) s_base = atomicAdd(&ep.app[e], staged);
         __syncthreads();
         for (uint32_t r = threadIdx.x; r < staged; r += blockDim.x) {
