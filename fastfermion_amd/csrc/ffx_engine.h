@@ -29,6 +29,7 @@
 #include "ffx_layer.h"
 #include "ffx_prune.h"
 #include "ffx_dlayer.h"
+#include "ffx_merged.h"
 
 using namespace ffx;
 
@@ -87,6 +88,8 @@ struct ffx_ctx {
     uint32_t* topk_buf = nullptr;  // top-k select scratch: 256 bins, prefix, rank, tie tickets
     void* batch_pool = nullptr;    // batched instances: term stores, schedules, states, inputs
     size_t batch_pool_bytes = 0;
+    void* merged_buf = nullptr;    // merged batch: windows, schedule offsets, states, results
+    size_t merged_buf_bytes = 0;
     size_t epochs_cap = 0;
     int n_cu = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -104,6 +107,7 @@ struct ffx_ctx {
         int W = 0, prec = 0, K = 0;
         int block = 0, window = 0;
         int64_t slots = 0;
+        int merged = 0;
     };
     std::vector<BatchCfg> batch_tuned;
     void* batch_host = nullptr;  // pinned
@@ -1941,7 +1945,222 @@ struct BatchConf {
     int block = 0;
     int window = 0;
     int64_t slots = 0;
+    int merged = -1;  // 1: one term store for all instances (ffx_merged.h), 0: one workgroup each
 };
+
+// Merged batch (ffx_merged.h): the instances elig[0..ne) in one term store, one
+// launch per window step of the longest schedule, and the device-decided
+// compaction every compact_every() / 32 steps (default 8: with truncation after
+// every rotation of 512 kicked-Ising instances, every 64 / 16 / 8 / 4 / 1 steps
+// take 250 / 188 / 181 / 183 / 209 ms).  `ran` stays false (nothing done) when the
+// instance numbers do not fit above every qubit the instances use.
+template <int W, class CP, int K>
+int run_pauli_merged(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<int32_t>& elig,
+                     const std::vector<std::vector<BStep<PauliAlg<W>, CP>>>& sched, double* out_expect,
+                     int64_t* out_terms, bool& ran) {
+    using Alg = PauliAlg<W>;
+    using Eng = Engine<Alg, CP, K>;
+    using store_t = typename CP::store_t;
+    using Win = Window<Alg, CP>;
+    constexpr int NP = Alg::NP;
+    ran = false;
+    const size_t ne = elig.size();
+    if (ne == 0) return FFX_OK;
+    int idbits = 1;
+    while (((size_t)1 << idbits) < ne) ++idbits;
+    uint64_t used[W] = {};
+    for (int32_t i : elig) {
+        const ffx_instance& in = inst[i];
+        for (int64_t g = 0; g < in.n_gates; ++g)
+            for (int w = 0; w < W; ++w) used[w] |= in.gates[g].x[w] | in.gates[g].z[w];
+        for (int64_t t = 0; t < in.n_terms; ++t)
+            for (int w = 0; w < W; ++w) used[w] |= in.x[t * W + w] | in.z[t * W + w];
+    }
+    int top = 0;
+    for (int w = 0; w < W; ++w)
+        if (used[w]) top = 64 * w + 64 - __builtin_clzll(used[w]);
+    if (top > 64 * W - idbits) return FFX_OK;
+    const uint32_t idshift = (uint32_t)(64 - idbits);
+    const uint64_t idmask = (1ull << idbits) - 1;
+    const uint64_t idbits_word = idmask << idshift;
+    // observable terms, the instance number in the top bits of the last Z word
+    size_t nt = 0;
+    for (int32_t i : elig) nt += (size_t)inst[i].n_terms;
+    std::vector<uint64_t> hx(nt * W), hz(nt * W);
+    std::vector<double> hre(nt), him(nt);
+    std::vector<uint64_t> hstate(ne * W);
+    std::vector<uint32_t> wbase(ne), wlen(ne);
+    size_t nw = 0, o = 0;
+    uint32_t maxlen = 0;
+    int64_t n_rot = 0;
+    for (size_t b = 0; b < ne; ++b) {
+        const ffx_instance& in = inst[elig[b]];
+        for (int64_t t = 0; t < in.n_terms; ++t, ++o) {
+            for (int w = 0; w < W; ++w) {
+                hx[o * W + w] = in.x[t * W + w];
+                hz[o * W + w] = in.z[t * W + w];
+            }
+            hz[o * W + W - 1] |= (uint64_t)b << idshift;
+            hre[o] = in.c_re[t];
+            him[o] = (K == 2 && in.c_im) ? in.c_im[t] : 0.0;
+        }
+        for (int w = 0; w < W; ++w) hstate[b * W + w] = in.basis_state[w] & (w == W - 1 ? ~idbits_word : ~0ull);
+        wbase[b] = (uint32_t)nw;
+        wlen[b] = (uint32_t)sched[elig[b]].size();
+        nw += wlen[b];
+        maxlen = std::max(maxlen, wlen[b]);
+    }
+    std::vector<Win> hw(std::max<size_t>(nw, 1));
+    for (size_t b = 0; b < ne; ++b)
+        for (uint32_t q = 0; q < wlen[b]; ++q) {
+            Win w = sched[elig[b]][q].w;
+            if (w.maxdeg < (1 << 29)) w.maxdeg += __builtin_popcountll((unsigned long long)b);
+            n_rot += w.k;
+            hw[wbase[b] + q] = w;
+        }
+    // device buffer: windows, wbase, wlen, states, sums, counts, work stripes
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_wbase = al(hw.size() * sizeof(Win));
+    const size_t o_wlen = o_wbase + al(ne * 4);
+    const size_t o_state = o_wlen + al(ne * 4);
+    const size_t o_out = o_state + al(ne * W * 8);
+    const size_t o_cnt = o_out + al(2 * ne * 8);
+    const size_t o_work = o_cnt + al(ne * 4);
+    const size_t total = o_work + al((size_t)kStripes * 16 * 8);
+    if (total > ctx->merged_buf_bytes) {
+        dev_free(ctx->merged_buf);
+        ctx->merged_buf_bytes = 0;
+        CHK(dev_alloc(ctx, &ctx->merged_buf, total));
+        ctx->merged_buf_bytes = total;
+    }
+    char* mb = (char*)ctx->merged_buf;
+    HIPCHK(hipMemcpyAsync(mb, hw.data(), hw.size() * sizeof(Win), hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(mb + o_wbase, wbase.data(), ne * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(mb + o_wlen, wlen.data(), ne * 4, hipMemcpyHostToDevice, ctx->stream));
+    HIPCHK(hipMemcpyAsync(mb + o_state, hstate.data(), ne * W * 8, hipMemcpyHostToDevice, ctx->stream));
+    Merged<Alg, CP> M{};
+    M.wins = (const Win*)mb;
+    M.wbase = (const uint32_t*)(mb + o_wbase);
+    M.wlen = (const uint32_t*)(mb + o_wlen);
+    M.idshift = idshift;
+    M.idmask = idmask;
+    M.work = (unsigned long long*)(mb + o_work);
+    M.states = (const uint64_t*)(mb + o_state);
+    M.out = (double*)(mb + o_out);
+    M.count = (unsigned int*)(mb + o_cnt);
+    Eng eng(ctx);
+    const uint32_t every = (uint32_t)std::max(1, compact_every() / 32);
+    if (ctx->n_cu <= 0) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+        ctx->n_cu = prop.multiProcessorCount;
+    }
+    const unsigned win_grid = 6 * (unsigned)ctx->n_cu;
+    size_t want = std::max<size_t>(4 * nt + 4096, 1 << 16);
+    std::vector<uint32_t> hn, hd, ha;
+    std::vector<int32_t> hdd;
+    uint32_t e = 1;
+    int n_compact = 0;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt >= 24) return set_err(ctx, FFX_ECAPACITY, "merged batch: term store keeps overflowing");
+        CHK(eng.upload(hx.data(), hz.data(), hre.data(), K == 2 ? him.data() : nullptr, (int64_t)nt));
+        CHK(eng.reserve(want));
+        const size_t n_ep = 2 + (size_t)maxlen + maxlen / every + 1;
+        CHK(eng.ensure_epochs(n_ep));
+        CHK(eng.ensure_counts(ctx->cap / kTile + 2));
+        const Epochs ep = eng.epochs();
+        HIPCHK(hipMemsetAsync(ctx->epochs, 0, Eng::epoch_words(ctx->epochs_cap) * sizeof(uint32_t), ctx->stream));
+        HIPCHK(hipMemsetAsync(mb + o_work, 0, (size_t)kStripes * 16 * 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(mb + o_out, 0, 2 * ne * 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(mb + o_cnt, 0, ne * 4, ctx->stream));
+        const uint32_t init[2] = {(uint32_t)nt, 0u};
+        HIPCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        const unsigned g_cap = grid_for(ctx->cap, kTile, 4096);
+        const unsigned g_idx = grid_for(std::max(ctx->cap, ctx->hslots), kBlock, 2048);
+        e = 1;
+        n_compact = 0;
+        for (uint32_t s = 0; s < maxlen; ++s) {
+            M.step = s;
+            hipLaunchKernelGGL((k_mwin_async<Alg, CP, K>), dim3(win_grid), dim3(kBlockWin), 0, ctx->stream,
+                               eng.store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, M, ep, e);
+            CHK(eng.launch_check());
+            std::swap(ctx->coef, ctx->coef2);
+            ++e;
+            if ((s + 1) % every == 0 && s + 1 < maxlen) {
+                const uint32_t ep_prev = e - 1;
+                hipLaunchKernelGGL((k_cmp_count<CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream, eng.store(), ep,
+                                   ep_prev, ctx->counts);
+                CHK(eng.launch_check());
+                hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(kBlock), 0, ctx->stream, ep, ep_prev, ctx->counts);
+                CHK(eng.launch_check());
+                hipLaunchKernelGGL((k_cmp_scatter<NP, CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream,
+                                   eng.store(), ctx->key2, (store_t*)ctx->coef2, ep, ep_prev, e, ctx->counts);
+                CHK(eng.launch_check());
+                hipLaunchKernelGGL((k_cmp_copyback<NP, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream,
+                                   eng.store(), ctx->key2, (const store_t*)ctx->coef2, ep, ep_prev, e);
+                CHK(eng.launch_check());
+                hipLaunchKernelGGL(k_index_build_async<NP>, dim3(g_idx), dim3(kBlock), 0, ctx->stream, ctx->key,
+                                   ctx->cap, ctx->index, ctx->hslots - 1, ep, ep_prev, e, 1, &ctx->ctrl->index_full);
+                CHK(eng.launch_check());
+                ++e;
+                ++n_compact;
+            }
+        }
+        hn.assign(e, 0);
+        hd.assign(e, 0);
+        ha.assign(e, 0);
+        hdd.assign(e, 0);
+        uint32_t hf[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(hn.data(), ep.n, e * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hd.data(), ep.dead, e * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ha.data(), ep.app, e * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hdd.data(), ep.dd, e * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hf, ep.flags, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->ctrl_host->index_full)
+            return set_err(ctx, FFX_EHIP, "internal error: hash index full (flags " +
+                                              std::to_string(ctx->ctrl_host->index_full) + ")");
+        if (!hf[0]) break;
+        want = 2 * ctx->cap;  // an append did not fit: a larger store, from the start
+    }
+    // slots and erased slots after the last launch, as the next launch would see them
+    const uint32_t n_fin = hn[e - 1] + ha[e - 1];
+    int64_t peak = 0;
+    for (uint32_t q = 1; q < e; ++q)
+        peak = std::max<int64_t>(peak, (int64_t)(hn[q] + ha[q]) - (int64_t)((int32_t)hd[q] + hdd[q]));
+    if (n_fin > 0) {
+        hipLaunchKernelGGL((k_mexpect<Alg, CP, K>), dim3(grid_for(n_fin, kBlock, 4096)), dim3(kBlock), 0, ctx->stream,
+                           eng.store(), n_fin, (const uint64_t*)(mb + o_state), idshift, idmask, (double*)(mb + o_out),
+                           (unsigned int*)(mb + o_cnt));
+        CHK(eng.launch_check());
+    }
+    std::vector<double> hout(2 * ne);
+    std::vector<uint32_t> hcnt(ne);
+    std::vector<unsigned long long> hwork((size_t)kStripes * 16);
+    HIPCHK(hipMemcpyAsync(hout.data(), mb + o_out, 2 * ne * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hcnt.data(), mb + o_cnt, ne * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hwork.data(), mb + o_work, hwork.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (size_t b = 0; b < ne; ++b) {
+        out_expect[2 * elig[b]] = hout[2 * b];
+        out_expect[2 * elig[b] + 1] = hout[2 * b + 1];
+        if (out_terms) out_terms[elig[b]] = (int64_t)hcnt[b];
+    }
+    for (int c = 0; c < kStripes; ++c) {
+        ctx->prof.updates += (int64_t)hwork[(size_t)c * 16];
+        ctx->prof.outs += (int64_t)hwork[(size_t)c * 16 + 1];
+    }
+    ctx->prof.n_gates += n_rot;
+    ctx->prof.peak_terms = std::max(ctx->prof.peak_terms, peak);
+    ctx->prof.run_launches += maxlen;
+    ctx->prof.compactions += n_compact;
+    // the store holds every instance's terms: no device-resident result of one call
+    ctx->last_kind = 0;
+    ran = true;
+    return FFX_OK;
+}
 
 template <int W, class CP, int K>
 int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, const ffx_options& o, int64_t slots,
@@ -1955,14 +2174,16 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     const int maxdeg = degree_limit(o.maxdegree);
     // configuration: explicit > autotuned for this (W, precision, K) > defaults
     const int prec_id = std::is_same<CP, F64>::value ? FFX_FP64 : (std::is_same<CP, F32>::value ? FFX_FP32 : FFX_FP32C);
-    BatchConf cf{kBlockBatch, kWin, 32768};
+    BatchConf cf{kBlockBatch, kWin, 32768, 1};
     for (const auto& t : ctx->batch_tuned)
-        if (t.W == W && t.prec == prec_id && t.K == K) cf = BatchConf{t.block, t.window, t.slots};
+        if (t.W == W && t.prec == prec_id && t.K == K) cf = BatchConf{t.block, t.window, t.slots, t.merged};
     if (conf) {
         if (conf->block) cf.block = conf->block;
         if (conf->window) cf.window = conf->window;
         if (conf->slots) cf.slots = conf->slots;
+        if (conf->merged >= 0) cf.merged = conf->merged;
     }
+    if (const char* ev = getenv("FFX_BATCH_MERGED")) cf.merged = atoi(ev) != 0 ? 1 : 0;
     if (slots > 0) cf.slots = slots;
     if (const char* ev = getenv("FFX_BATCH_BLOCK")) {
         const int v = atoi(ev);
@@ -2021,8 +2242,14 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     for (int32_t i = 0; i < n_inst; ++i)
         if (ok[i]) elig.push_back(i);
     std::stable_sort(elig.begin(), elig.end(), [&](int32_t a, int32_t b) { return est[a] > est[b]; });
-    const size_t ne = elig.size();
-    int32_t fallback = n_inst - (int32_t)ne;
+    bool merged_ran = false;
+    if (cf.merged == 1 && !elig.empty()) {
+        CHK((run_pauli_merged<W, CP, K>(ctx, inst, elig, sched, out_expect, out_terms, merged_ran)));
+        if (merged_ran)
+            for (int32_t i : elig) done[i] = 1;
+    }
+    const size_t ne = merged_ran ? 0 : elig.size();
+    int32_t fallback = n_inst - (int32_t)elig.size();
     if (ne > 0) {
         // one pool: per-instance stores, then the states, schedules and inputs
         const size_t hs = 2 * S;

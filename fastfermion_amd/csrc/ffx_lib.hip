@@ -68,6 +68,7 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     dev_free(ctx->epochs);
     dev_free(ctx->topk_buf);
     dev_free(ctx->batch_pool);
+    dev_free(ctx->merged_buf);
     dev_free(ctx->crot);
     dev_free(ctx->claim);
     dev_free(ctx->layer_buf);
@@ -390,28 +391,31 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     const int blocks[3] = {128, 256, 512};
     const int windows[2] = {3, 4};
     const int64_t budgets[2] = {8192, 32768};
+    // candidates: the merged layout per window length, then the one-workgroup layout
+    std::vector<BatchConf> cands;
+    for (int wi : windows) cands.push_back(BatchConf{kBlockBatch, wi, 32768, 1});
+    for (int bl : blocks)
+        for (int wi : windows)
+            for (int64_t sl : budgets) cands.push_back(BatchConf{bl, wi, sl, 0});
     double best_ms = 1e300;
     BatchConf best{};
     int32_t tried = 0;
     bool best_clean = false;
-    for (int bl : blocks)
-        for (int wi : windows)
-            for (int64_t sl : budgets) {
-                const BatchConf c{bl, wi, sl};
-                int32_t fb = 0;
-                // first run builds and captures; the second is timed
-                CHK(batch_impl(ctx, sub.data(), m, n_words, *opts, 0, ex.data(), nullptr, &fb, K, &c));
-                const double t0 = now_s();
-                CHK(batch_impl(ctx, sub.data(), m, n_words, *opts, 0, ex.data(), nullptr, &fb, K, &c));
-                const double ms = 1e3 * (now_s() - t0);
-                ++tried;
-                const bool clean = fb == 0;
-                if ((clean && !best_clean) || (clean == best_clean && ms < best_ms)) {
-                    best_ms = ms;
-                    best = c;
-                    best_clean = clean;
-                }
-            }
+    for (const BatchConf& c : cands) {
+        int32_t fb = 0;
+        // first run builds (and captures); the second is timed
+        CHK(batch_impl(ctx, sub.data(), m, n_words, *opts, 0, ex.data(), nullptr, &fb, K, &c));
+        const double t0 = now_s();
+        CHK(batch_impl(ctx, sub.data(), m, n_words, *opts, 0, ex.data(), nullptr, &fb, K, &c));
+        const double ms = 1e3 * (now_s() - t0);
+        ++tried;
+        const bool clean = fb == 0;
+        if ((clean && !best_clean) || (clean == best_clean && ms < best_ms)) {
+            best_ms = ms;
+            best = c;
+            best_clean = clean;
+        }
+    }
     ffx_ctx::BatchCfg t;
     t.W = n_words;
     t.prec = opts->precision;
@@ -419,6 +423,7 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     t.block = best.block;
     t.window = best.window;
     t.slots = best.slots;
+    t.merged = best.merged;
     bool replaced = false;
     for (auto& e : ctx->batch_tuned)
         if (e.W == t.W && e.prec == t.prec && e.K == t.K) {
@@ -429,6 +434,8 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     out->block = best.block;
     out->window = best.window;
     out->slots = best.slots;
+    out->merged = best.merged;
+    out->reserved = 0;
     out->ms = best_ms;
     out->candidates = tried;
     out->sample = m;

@@ -170,16 +170,25 @@ typedef struct ffx_instance {
 /* Batched Pauli propagation (no reference counterpart: the reference runs one
  * circuit per propagate call, src/pauli/propagate.h:297-493, and a batch is a
  * loop over such calls).  Every instance runs the reference schedule and
- * truncation of ffx_pauli_propagate with the shared options; all instances
- * run in one launch, one workgroup each (largest estimated work first:
- * rotations^2 / qubits), with `slots` term slots per instance
- * (0: a default).  Instances whose terms outgrow that budget, that contain
- * Clifford gates or that ask for top-k are run afterwards on the device-wide
- * engine, so every result is complete.  The upload, the launch and the download
- * of the instances' states run as one captured HIP graph, replayed when the next
- * call has the same layout.  out_expect[2i], [2i+1] receives
- * <b_i|O_i|b_i>; out_terms[i] (optional) the live term count at the end;
- * out_fallback (optional) how many instances took the device-wide engine. */
+ * truncation of ffx_pauli_propagate with the shared options, in one of two
+ * layouts (the autotuned one, else merged):
+ *  - merged: all instances' terms in one device-wide term store, each tagged
+ *    with its instance number in key bits above the instances' qubits; one
+ *    launch per window step applies every instance's own window (the longest
+ *    schedule sets the launch count).  Needs ceil(log2 n_inst) free bits at
+ *    the top of the last key word; otherwise the other layout runs.
+ *  - one workgroup per instance: all instances in one launch (largest
+ *    estimated work first: rotations^2 / qubits), with `slots` term slots per
+ *    instance (0: a default); the upload, the launch and the download of the
+ *    instances' states run as one captured HIP graph, replayed when the next
+ *    call has the same layout.  Instances whose terms outgrow the slot budget
+ *    run afterwards on the device-wide engine.
+ * Instances that contain Clifford gates or ask for top-k run on the device-wide
+ * engine, so every result is complete.  out_expect[2i], [2i+1] receives
+ * <b_i|O_i|b_i> (merged: summed with atomics, so the last bits can vary from
+ * run to run); out_terms[i] (optional) the live term count at the end;
+ * out_fallback (optional) how many instances took the device-wide engine.
+ * FFX_BATCH_MERGED=0 / 1 in the environment forces a layout. */
 int ffx_pauli_propagate_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
                               const ffx_options* opts, int64_t slots, double* out_expect, int64_t* out_terms,
                               int32_t* out_fallback);
@@ -192,11 +201,14 @@ typedef struct ffx_batch_tuning {
     double ms;           /* time of the sample batch with this configuration */
     int32_t candidates;  /* configurations timed */
     int32_t sample;      /* instances in the sample batch */
+    int32_t merged;      /* 1: merged layout (block and slots unused), 0: one workgroup per instance */
+    int32_t reserved;
 } ffx_batch_tuning;
 
-/* Autotuner of batched propagation (no reference counterpart): times every
- * combination of workgroup size, window length and slot budget on the whole
- * batch (sample <= 0) or on the `sample` instances with the longest circuits,
+/* Autotuner of batched propagation (no reference counterpart): times the
+ * merged layout with windows of 3 and 4 rotations and every combination of
+ * workgroup size, window length and slot budget of the one-workgroup layout on
+ * the whole batch (sample <= 0) or on the `sample` instances with the longest circuits,
  * with the options of `opts`; prefers configurations without device-wide
  * fallbacks, and keeps the fastest on the context for this key width, precision
  * and coefficient kind; later ffx_pauli_propagate_batch calls with slots = 0

@@ -1,9 +1,11 @@
-"""Batched propagation (one workgroup per circuit instance, ffx_pauli_propagate_batch)
-against the oracle run instance by instance, and against the device-wide
-engine.  The batch runs each instance's reference schedule and truncation, so
-FP64 expectations agree with the oracle to rounding of the final sum and the
-live term counts and work counters are identical; instances that outgrow their
-slot budget, contain Clifford gates or ask for top-k must come back complete
+"""Batched propagation (ffx_pauli_propagate_batch) in both layouts -- merged (all
+instances in one term store, instance numbers in the top key bits, one launch
+per window step) and one workgroup per instance -- against the oracle run
+instance by instance, and against the device-wide engine.  The batch runs each
+instance's reference schedule and truncation, so FP64 expectations agree with
+the oracle to rounding of the final sum and the live term counts and work
+counters are identical; instances that outgrow their slot budget (one-workgroup
+layout), contain Clifford gates or ask for top-k must come back complete
 through the device-wide engine."""
 from __future__ import annotations
 
@@ -22,6 +24,12 @@ def ctx():
     c = Context(0)
     yield c
     c.close()
+
+
+@pytest.fixture(params=["merged", "workgroup"])
+def layout(request, monkeypatch):
+    monkeypatch.setenv("FFX_BATCH_MERGED", "1" if request.param == "merged" else "0")
+    return request.param
 
 
 def pad(a, W):
@@ -52,7 +60,7 @@ def run_both(ctx, inst, W, states=None, **kw):
 
 
 @pytest.mark.parametrize("batched", [False, True])
-def test_batch_matches_oracle_per_instance(ctx, batched):
+def test_batch_matches_oracle_per_instance(ctx, layout, batched):
     inst = instances(24, 11, l_lo=3, l_span=4, d_lo=2, d_span=5)
     W = 1
     kw = dict(maxdegree=8, mincoeff=1e-4, batched=batched)
@@ -70,7 +78,7 @@ def test_batch_matches_oracle_per_instance(ctx, batched):
     assert ctx.profile()["updates"] == upd
 
 
-def test_batch_uneven_two_words_and_states(ctx):
+def test_batch_uneven_two_words_and_states(ctx, layout):
     inst = instances(16, 5, l_lo=6, l_span=5, d_lo=3, d_span=6)
     W = 2
     rng = np.random.default_rng(3)
@@ -81,9 +89,11 @@ def test_batch_uneven_two_words_and_states(ctx):
         assert nt[i] == n
 
 
-def test_batch_slot_budget_fallback(ctx):
+def test_batch_slot_budget_fallback(ctx, monkeypatch):
     """A budget far below the instances' peaks: they stop in the batch launch
-    and rerun on the device-wide engine with identical results."""
+    and rerun on the device-wide engine with identical results (one-workgroup
+    layout; the merged store grows instead)."""
+    monkeypatch.setenv("FFX_BATCH_MERGED", "0")
     inst = instances(6, 17, l_lo=7, l_span=2, d_lo=8, d_span=3)
     W = 1
     kw = dict(maxdegree=8, mincoeff=1e-5, batched=True)
@@ -94,7 +104,7 @@ def test_batch_slot_budget_fallback(ctx):
         assert nt[i] == n
 
 
-def test_batch_ineligible_instances(ctx):
+def test_batch_ineligible_instances(ctx, layout):
     """Clifford gates and top-k go to the device-wide engine; mixed with normal
     instances in one call."""
     inst = instances(4, 23, l_lo=3, l_span=2, d_lo=2, d_span=3)
@@ -115,7 +125,7 @@ def test_batch_ineligible_instances(ctx):
 
 
 @pytest.mark.parametrize("prec,tol", [(_abi.FP32C, 1e-9), (_abi.FP32, 1e-4)])
-def test_batch_precisions(ctx, prec, tol):
+def test_batch_precisions(ctx, layout, prec, tol):
     inst = instances(12, 29, l_lo=4, l_span=4, d_lo=3, d_span=6)
     W = 1
     items = [(g, x, z, c, None) for (g, x, z, c, L, d) in inst]
@@ -125,7 +135,7 @@ def test_batch_precisions(ctx, prec, tol):
     assert np.max(np.abs(e64 - e32)) < tol
 
 
-def test_batch_equals_single_engine(ctx):
+def test_batch_equals_single_engine(ctx, layout):
     inst = instances(5, 31, l_lo=5, l_span=3, d_lo=4, d_span=4)
     W = 1
     items = [(g, x, z, c, None) for (g, x, z, c, L, d) in inst]
@@ -137,7 +147,7 @@ def test_batch_equals_single_engine(ctx):
         assert nt[i] == len(res)
 
 
-def test_batch_empty_and_trivial(ctx):
+def test_batch_empty_and_trivial(ctx, layout):
     ex, nt, fb = ctx.pauli_propagate_batch([], 1)
     assert len(ex) == 0 and fb == 0
     x, z, c = wl.single_pauli("Z", [0], 4)
@@ -148,9 +158,11 @@ def test_batch_empty_and_trivial(ctx):
 def test_graph_replays_match_direct_launches(ctx, monkeypatch):
     """A batch whose pool layout repeats (same instances, new angles: an angle
     sweep) replays the captured HIP graph.  Every replay, with eager work and
-    synchronisations between them, must equal the direct launch bit for bit."""
+    synchronisations between them, must equal the direct launch bit for bit
+    (one-workgroup layout, the one that is captured)."""
     import math
 
+    monkeypatch.setenv("FFX_BATCH_MERGED", "0")
     W = 1
     base = instances(32, 21, l_lo=3, l_span=4, d_lo=2, d_span=6)
     kw = dict(maxdegree=8, mincoeff=1e-4, batched=False)
@@ -180,8 +192,9 @@ def test_graph_replays_match_direct_launches(ctx, monkeypatch):
 
 
 def test_autotune_keeps_results(ctx):
-    """The autotuned launch (workgroup size, window length, slot budget) changes
-    only speed: FP64 results stay identical to the default launch."""
+    """The autotuned launch (layout, workgroup size, window length, slot budget)
+    changes only speed: FP64 term counts stay identical to the default launch and
+    expectations agree to rounding of their sums."""
     W = 1
     inst = instances(40, 31, l_lo=3, l_span=5, d_lo=3, d_span=8)
     items = [(g, pad(x, W), pad(z, W), c, None) for (g, x, z, c, L, d) in inst]
@@ -189,6 +202,7 @@ def test_autotune_keeps_results(ctx):
     want = ctx.pauli_propagate_batch(items, W, **kw)
     t = ctx.batch_autotune(items, W, sample=16, **kw)
     assert t["block"] in (128, 256, 512) and 1 <= t["window"] <= 4 and t["slots"] >= 1024
-    assert t["candidates"] == 12 and t["sample"] == 16 and t["ms"] > 0
+    assert t["merged"] in (0, 1)
+    assert t["candidates"] == 14 and t["sample"] == 16 and t["ms"] > 0
     got = ctx.pauli_propagate_batch(items, W, **kw)
-    assert np.array_equal(got[0], want[0]) and np.array_equal(got[1], want[1])
+    assert np.max(np.abs(got[0] - want[0])) < 1e-13 and np.array_equal(got[1], want[1])
