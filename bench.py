@@ -279,17 +279,40 @@ def side_measurements(ctx, gates, W, x, z, c, plain_ms, plain_expect, cpu_legs=T
     tune_s = time.perf_counter() - ta
     r0 = ctx.batch_graph_replays()
     dt32, (e32, _, fb) = _time(lambda: ctx.pauli_propagate_batch(items, Wb, precision=_abi.FP32C, **kw), reps=3)
-    upd = ctx.profile()["updates"]
-    graph_ms = 1e3 * ctx.profile()["seconds_kernel"]
+    prof = ctx.profile()
+    upd = prof["updates"]
+    graph_ms = 1e3 * prof["seconds_kernel"]
     replays = ctx.batch_graph_replays() - r0
     d = np.abs(e32 - e64)
+    # the heaviest instances (by estimated work) alone on the device-wide engine, FP32
+    # compensated as the batch, on a context of their own (whose term store fits them):
+    # the longest of them is the batch's lower bound
+    from fastfermion_amd._native import Context
+
+    solo = Context(0)
+    longest = []
+    for i in np.argsort([-(len(g) ** 2) / (L * L) for (g, _, _, _, L, _) in inst])[:4]:
+        g, xi, zi, ci, L, dpt = inst[int(i)]
+        Wi = xi.shape[1]
+        dti, _ = _time(lambda: solo.pauli_propagate(g, Wi, xi, zi, ci, download=False, want_expect=True,
+                                                    precision=_abi.FP32C, **kw), reps=3)
+        longest.append({"instance": int(i), "L": int(L), "depth": int(dpt), "rotations": int(len(g)),
+                        "ms": 1e3 * dti, "launches": solo.profile()["run_launches"]})
+    solo.close()
+    longest_ms = max(x["ms"] for x in longest)
+    layout = ("merged: every instance's terms in one store, per layer-group step one claim launch and one "
+              "replay launch (orbits spread over the grid)" if tuned.get("merged") else
+              "one workgroup per instance, one launch")
     out["batch"] = {"instances": len(items), "workload": "kicked-Ising L in 6..12, depth 4..24, RX angles in "
-                    "[0, pi/2], weight-1/2 Z observable, maxdegree 8, mincoeff 1e-4, per-rotation truncation, "
-                    "one workgroup per instance, one launch", "precision": "fp32c", "ms": 1e3 * dt32,
+                    "[0, pi/2], weight-1/2 Z observable, maxdegree 8, mincoeff 1e-4, per-rotation truncation",
+                    "layout": layout, "precision": "fp32c", "ms": 1e3 * dt32,
                     "updates": int(upd), "updates_per_s": upd / dt32, "fallback_instances": fb,
-                    "graph_ms": graph_ms, "fp64_ms": 1e3 * dt64, "max_abs_err_vs_fp64": float(d.max()),
+                    "launches": int(prof["run_launches"]), "graph_ms": graph_ms, "fp64_ms": 1e3 * dt64,
+                    "max_abs_err_vs_fp64": float(d.max()),
                     "median_abs_err_vs_fp64": float(np.median(d)), "default_launch_ms": 1e3 * dt_default,
-                    "autotuned": dict(tuned, seconds=tune_s), "graph_replays_of_3_timed": replays}
+                    "autotuned": dict(tuned, seconds=tune_s), "graph_replays_of_3_timed": replays,
+                    "heaviest_instances_alone": longest,
+                    "ratio_to_longest_instance": 1e3 * dt32 / longest_ms}
     return out
 
 

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <array>
 #include <atomic>
 #include <chrono>
 #include <cmath>
@@ -30,6 +31,7 @@
 #include "ffx_prune.h"
 #include "ffx_dlayer.h"
 #include "ffx_merged.h"
+#include "ffx_mlayer.h"
 
 using namespace ffx;
 
@@ -88,8 +90,13 @@ struct ffx_ctx {
     uint32_t* topk_buf = nullptr;  // top-k select scratch: 256 bins, prefix, rank, tie tickets
     void* batch_pool = nullptr;    // batched instances: term stores, schedules, states, inputs
     size_t batch_pool_bytes = 0;
-    void* merged_buf = nullptr;    // merged batch: windows, schedule offsets, states, results
+    void* merged_buf = nullptr;    // merged batch: windows or groups, schedule offsets, states, results
     size_t merged_buf_bytes = 0;
+    void* mlayer_host = nullptr;   // merged batch in layer groups: pinned image of the device buffer
+    size_t mlayer_host_bytes = 0;
+    void* mlayer_q = nullptr;      // merged batch in layer groups: orbit leader lists, their counts
+    size_t mlayer_q_bytes = 0;
+    size_t mlayer_cap_hint = 0;    // merged batch in layer groups: store capacity of the last call
     size_t epochs_cap = 0;
     int n_cu = 0;
     hipEvent_t ev0 = nullptr, ev1 = nullptr;
@@ -503,18 +510,23 @@ struct Engine {
     }
 
     // Make room for `need` slots; existing slots [0, n_host) are preserved.
-    int reserve(size_t need) {
-        if (need <= ctx->cap && ctx->key_planes == NP && ctx->coef_planes == coef_planes() &&
-            ctx->coef_elem == sizeof(store_t))
-            return FFX_OK;
+    // Store of at least `need` slots.  With `shrink`, a store that an earlier, larger
+    // run left far larger than `need` is reallocated at this size: kernels that sweep
+    // the whole index (its rebuild after every compaction) cost in proportion to the
+    // capacity.
+    int reserve(size_t need, bool shrink = false) {
         if (need > 0x7FFFFFF0ull) return set_err(ctx, FFX_ECAPACITY, "term count exceeds 2^31 slots");
         size_t ncap = pow2ceil(std::max<size_t>(need, 1024));
+        const bool oversized = shrink && !ctx->arena.on && ctx->cap >= ((size_t)1 << 22) && ctx->cap >= 16 * ncap;
         const bool same_layout =
             ctx->key_planes == NP && ctx->coef_planes == coef_planes() && ctx->coef_elem == sizeof(store_t);
+        if (need <= ctx->cap && same_layout && !oversized) return FFX_OK;
         // growth of the same layout doubles (amortised overflow restarts); a layout change
         // (key width, precision, real/complex) keeps the capacity the context already had
-        if (same_layout && ncap < 2 * ctx->cap) ncap = std::max<size_t>(ncap, 2 * ctx->cap);
-        if (!same_layout) ncap = std::max<size_t>(ncap, ctx->cap);
+        if (!oversized) {
+            if (same_layout && ncap < 2 * ctx->cap) ncap = std::max<size_t>(ncap, 2 * ctx->cap);
+            if (!same_layout) ncap = std::max<size_t>(ncap, ctx->cap);
+        }
         if (ctx->arena.on) {  // one store of the planned size; no growth
             if (need > ctx->arena.cap || ctx->cap != 0)
                 return set_err(ctx, FFX_ECAPACITY,
@@ -2162,6 +2174,404 @@ int run_pauli_merged(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
     return FFX_OK;
 }
 
+// One instance's schedule cut into layer groups (ffx_mlayer.h): at each rotation
+// the longer of a run on pairwise disjoint supports (at most kMGroupMax) and a
+// run of pairwise commuting rotations with independent axes (at most
+// kLayerMaxA; a single rotation is one), the disjoint run on ties.  Offsets in
+// the records are local to the instance.  False if a step is not a rotation.
+template <int W, class CP>
+struct MInstPlan {
+    std::vector<MGroup<W>> groups;
+    std::vector<DLayerGate<PauliAlg<W>, CP>> gates;
+    std::vector<uint8_t> pos;
+};
+
+template <int W, class CP, int K>
+bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>& sd, MInstPlan<W, CP>& P) {
+    using Eng = Engine<PauliAlg<W>, CP, K>;
+    constexpr int NP = 2 * W;
+    size_t dmax = kMGroupMax, gmax = kLayerMaxA;
+    if (const char* ev = getenv("FFX_MGROUP_MAX"))
+        if (atoi(ev) > 0) dmax = std::min<size_t>(kMGroupMax, (size_t)atoi(ev));
+    if (const char* ev = getenv("FFX_MGENERAL_MAX"))
+        if (atoi(ev) > 0) gmax = std::min<size_t>(kLayerMaxA, (size_t)atoi(ev));
+    P.groups.clear();
+    P.gates.clear();
+    P.pos.clear();
+    for (const auto& a : sd)
+        if (a.kind != 0) return false;
+    P.gates.reserve(sd.size());
+    for (size_t t = 0; t < sd.size();) {
+        // run on pairwise disjoint supports
+        size_t kd = 0;
+        {
+            uint64_t used[4] = {0, 0, 0, 0};
+            while (t + kd < sd.size() && kd < dmax) {
+                uint64_t u[4];
+                Eng::axis_units(sd[t + kd].g, u);
+                int cnt = 0;
+                bool overlap = false;
+                for (int w = 0; w < 4; ++w) {
+                    cnt += __builtin_popcountll(u[w]);
+                    overlap = overlap || (u[w] & used[w]) != 0;
+                }
+                if (cnt == 0 || overlap) break;
+                for (int w = 0; w < 4; ++w) used[w] |= u[w];
+                ++kd;
+            }
+        }
+        // run of pairwise commuting rotations with linearly independent axes
+        size_t kg = 0;
+        {
+            uint64_t basis[kLayerMaxA][NP];
+            int pivot[kLayerMaxA];
+            int nb = 0;
+            while (t + kg < sd.size() && kg < gmax) {
+                const auto& a = sd[t + kg];
+                bool ok = true;
+                for (size_t j = 0; j < kg && ok; ++j) ok = Eng::axes_commute(sd[t + j].g, a.g);
+                if (!ok) break;
+                uint64_t v[NP];
+                for (int q = 0; q < NP; ++q) v[q] = a.g.p[q];
+                for (int b = 0; b < nb; ++b)
+                    if ((v[pivot[b] >> 6] >> (pivot[b] & 63)) & 1)
+                        for (int q = 0; q < NP; ++q) v[q] ^= basis[b][q];
+                int q0 = -1;
+                for (int q = 0; q < NP && q0 < 0; ++q)
+                    if (v[q]) q0 = q;
+                if (q0 < 0) break;  // dependent (or the identity)
+                pivot[nb] = 64 * q0 + __builtin_ctzll(v[q0]);
+                for (int q = 0; q < NP; ++q) basis[nb][q] = v[q];
+                ++nb;
+                ++kg;
+            }
+        }
+        const bool general = !(kd >= 2 && kd >= kg);
+        const size_t k = general ? std::max<size_t>(kg, 1) : kd;
+        MGroup<W> G{};
+        G.gate_off = (uint32_t)P.gates.size();
+        G.pos_off = (uint32_t)P.pos.size();
+        G.nr = (uint32_t)k;
+        G.general = general ? 1u : 0u;
+        uint8_t upos[64 * W];
+        for (size_t j = 0; j < k; ++j) {
+            const auto& a = sd[t + j];
+            DLayerGate<PauliAlg<W>, CP> r{};
+            r.g = a.g;
+            r.cs = make_fac<CP>(a.cs);
+            r.sn = make_fac<CP>(a.sn);
+            r.pivot = -1;
+            for (int q = 0; q < NP && r.pivot < 0; ++q)
+                if (a.g.p[q]) r.pivot = 64 * q + __builtin_ctzll(a.g.p[q]);
+            if (a.trunc_thresh) G.tmask |= 1ull << j;
+            if (a.trunc_xw) G.xmask |= 1ull << j;
+            uint64_t u[4];
+            Eng::axis_units(a.g, u);
+            for (int w = 0; w < W; ++w) {
+                G.umask[w] |= u[w];
+                if (!general)
+                    for (uint64_t b = u[w]; b; b &= b - 1) upos[64 * w + __builtin_ctzll(b)] = (uint8_t)j;
+            }
+            P.gates.push_back(r);
+        }
+        if (!general)
+            for (int w = 0; w < W; ++w)
+                for (uint64_t b = G.umask[w]; b; b &= b - 1) P.pos.push_back(upos[64 * w + __builtin_ctzll(b)]);
+        P.groups.push_back(G);
+        t += k;
+    }
+    return true;
+}
+
+// Merged batch in layer groups (ffx_mlayer.h): the instances elig[0..ne) in one
+// term store, one launch per group step of the longest schedule, the
+// device-decided compaction every FFX_MCOMPACT steps (default 4).  `ran` stays
+// false (nothing done) when the instance numbers do not fit above every qubit
+// the instances use (`used`: the union of their qubit masks).
+template <int W, class CP, int K>
+int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<int32_t>& elig,
+                     const std::vector<MInstPlan<W, CP>>& plans, const uint64_t (&used)[W], const ffx_options& o,
+                     double* out_expect, int64_t* out_terms, bool& ran) {
+    using Alg = PauliAlg<W>;
+    using Eng = Engine<Alg, CP, K>;
+    using store_t = typename CP::store_t;
+    using Gate = DLayerGate<Alg, CP>;
+    constexpr int NP = Alg::NP;
+    ran = false;
+    const size_t ne = elig.size();
+    if (ne == 0) return FFX_OK;
+    int idbits = 1;
+    while (((size_t)1 << idbits) < ne) ++idbits;
+    int top = 0;
+    for (int w = 0; w < W; ++w)
+        if (used[w]) top = 64 * w + 64 - __builtin_clzll(used[w]);
+    if (top > 64 * W - idbits) return FFX_OK;
+    const uint32_t idshift = (uint32_t)(64 - idbits);
+    const uint64_t idmask = (1ull << idbits) - 1;
+    const uint64_t idbits_word = idmask << idshift;
+    // sizes and per-instance offsets
+    std::vector<uint32_t> gbase(ne), glen(ne), gate_base(ne), pos_base(ne);
+    size_t ng = 0, ngate = 0, npos = 0, nt = 0;
+    uint32_t maxlen = 0;
+    int64_t n_rot = 0;
+    for (size_t b = 0; b < ne; ++b) {
+        const MInstPlan<W, CP>& P = plans[elig[b]];
+        gbase[b] = (uint32_t)ng;
+        glen[b] = (uint32_t)P.groups.size();
+        gate_base[b] = (uint32_t)ngate;
+        pos_base[b] = (uint32_t)npos;
+        ng += P.groups.size();
+        ngate += P.gates.size();
+        npos += P.pos.size();
+        nt += (size_t)inst[elig[b]].n_terms;
+        maxlen = std::max(maxlen, glen[b]);
+        n_rot += (int64_t)P.gates.size();
+    }
+    // device buffer: groups, gates, positions, gbase, glen, states | sums, counts, work stripes
+    auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
+    const size_t o_gate = al(std::max<size_t>(ng, 1) * sizeof(MGroup<W>));
+    const size_t o_pos = o_gate + al(std::max<size_t>(ngate, 1) * sizeof(Gate));
+    const size_t o_gbase = o_pos + al(std::max<size_t>(npos, 1));
+    const size_t o_glen = o_gbase + al(ne * 4);
+    const size_t o_state = o_glen + al(ne * 4);
+    const size_t o_out = o_state + al(ne * W * 8);
+    const size_t o_cnt = o_out + al(2 * ne * 8);
+    const size_t o_work = o_cnt + al(ne * 4);
+    const size_t total = o_work + al((size_t)kStripes * 16 * 8);
+    if (total > ctx->merged_buf_bytes) {
+        dev_free(ctx->merged_buf);
+        ctx->merged_buf_bytes = 0;
+        CHK(dev_alloc(ctx, &ctx->merged_buf, total));
+        ctx->merged_buf_bytes = total;
+    }
+    if (o_out > ctx->mlayer_host_bytes) {
+        if (ctx->mlayer_host) (void)hipHostFree(ctx->mlayer_host);
+        ctx->mlayer_host = nullptr;
+        ctx->mlayer_host_bytes = 0;
+        HIPCHK(hipHostMalloc(&ctx->mlayer_host, o_out, hipHostMallocDefault));
+        ctx->mlayer_host_bytes = o_out;
+    }
+    char* H = (char*)ctx->mlayer_host;
+    {
+        // the pinned image, instance by instance on host threads (offsets made global)
+        const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+        std::atomic<size_t> next{0};
+        auto fill = [&]() {
+            for (size_t b = next++; b < ne; b = next++) {
+                const MInstPlan<W, CP>& P = plans[elig[b]];
+                MGroup<W>* hg = (MGroup<W>*)H + gbase[b];
+                for (size_t q = 0; q < P.groups.size(); ++q) {
+                    MGroup<W> G = P.groups[q];
+                    G.gate_off += gate_base[b];
+                    G.pos_off += pos_base[b];
+                    hg[q] = G;
+                }
+                std::memcpy(H + o_gate + (size_t)gate_base[b] * sizeof(Gate), P.gates.data(),
+                            P.gates.size() * sizeof(Gate));
+                if (!P.pos.empty()) std::memcpy(H + o_pos + pos_base[b], P.pos.data(), P.pos.size());
+                const ffx_instance& in = inst[elig[b]];
+                uint64_t* st = (uint64_t*)(H + o_state) + b * W;
+                for (int w = 0; w < W; ++w) st[w] = in.basis_state[w] & (w == W - 1 ? ~idbits_word : ~0ull);
+            }
+        };
+        std::vector<std::thread> pool;
+        const int nuse = (int)std::min<size_t>((size_t)nth, ne);
+        for (int t = 1; t < nuse; ++t) pool.emplace_back(fill);
+        fill();
+        for (auto& th : pool) th.join();
+        std::memcpy(H + o_gbase, gbase.data(), ne * 4);
+        std::memcpy(H + o_glen, glen.data(), ne * 4);
+    }
+    char* mb = (char*)ctx->merged_buf;
+    HIPCHK(hipMemcpyAsync(mb, H, o_out, hipMemcpyHostToDevice, ctx->stream));
+    // observable terms, the instance number in the top bits of the last Z word
+    std::vector<uint64_t> hx(nt * W), hz(nt * W);
+    std::vector<double> hre(nt), him(nt);
+    {
+        size_t q = 0;
+        for (size_t b = 0; b < ne; ++b) {
+            const ffx_instance& in = inst[elig[b]];
+            for (int64_t t = 0; t < in.n_terms; ++t, ++q) {
+                for (int w = 0; w < W; ++w) {
+                    hx[q * W + w] = in.x[t * W + w];
+                    hz[q * W + w] = in.z[t * W + w];
+                }
+                hz[q * W + W - 1] |= (uint64_t)b << idshift;
+                hre[q] = in.c_re[t];
+                him[q] = (K == 2 && in.c_im) ? in.c_im[t] : 0.0;
+            }
+        }
+    }
+    MLayer<Alg, CP> M{};
+    M.groups = (const MGroup<W>*)mb;
+    M.gates = (const Gate*)(mb + o_gate);
+    M.pos = (const uint8_t*)(mb + o_pos);
+    M.gbase = (const uint32_t*)(mb + o_gbase);
+    M.glen = (const uint32_t*)(mb + o_glen);
+    M.states = (const uint64_t*)(mb + o_state);
+    M.out = (double*)(mb + o_out);
+    M.count = (unsigned int*)(mb + o_cnt);
+    M.work = (unsigned long long*)(mb + o_work);
+    M.idshift = idshift;
+    M.idmask = idmask;
+    M.mincoeff = o.mincoeff;
+    M.maxdeg = degree_limit(o.maxdegree);
+    M.max_xw = o.max_xweight;
+    Eng eng(ctx);
+    uint32_t every = 4;
+    if (const char* ev = getenv("FFX_MCOMPACT"))
+        if (atoi(ev) > 0) every = (uint32_t)atoi(ev);
+    if (ctx->n_cu <= 0) {
+        hipDeviceProp_t prop;
+        HIPCHK(hipGetDeviceProperties(&prop, ctx->device));
+        ctx->n_cu = prop.multiProcessorCount;
+    }
+    unsigned grid = 6 * (unsigned)ctx->n_cu;
+    if (const char* ev = getenv("FFX_WIN_GRID"))
+        if (atoi(ev) > 0) grid = (unsigned)atoi(ev);
+    // the store: as large as the last call's (a batch of the same shape needs the same),
+    // doubled from the start whenever an append does not fit
+    size_t want = std::max<size_t>(std::max<size_t>(4 * nt + 4096, 1 << 16), ctx->mlayer_cap_hint);
+    std::vector<uint32_t> hn, hd, ha;
+    std::vector<int32_t> hdd;
+    uint32_t e = 1;
+    int n_compact = 0;
+    for (int attempt = 0;; ++attempt) {
+        if (attempt >= 24) return set_err(ctx, FFX_ECAPACITY, "merged batch: term store keeps overflowing");
+        CHK(eng.upload(hx.data(), hz.data(), hre.data(), K == 2 ? him.data() : nullptr, (int64_t)nt));
+        CHK(eng.reserve(want, true));
+        if (ctx->claim_slots != ctx->hslots || !ctx->claim) {
+            dev_free(ctx->claim);
+            ctx->claim_slots = 0;
+            CHK(dev_alloc(ctx, &ctx->claim, ctx->hslots * sizeof(uint64_t)));
+            ctx->claim_slots = ctx->hslots;
+            HIPCHK(hipMemsetAsync(ctx->claim, 0, ctx->hslots * sizeof(uint64_t), ctx->stream));
+            ctx->layer_stamp = 0;
+        }
+        M.claim = ctx->claim;
+        // orbit leader lists of the replay launches: a stripe's blocks hold at most
+        // ceil(grid / kMStripes) * (slots per block) leaders of each orbit size
+        {
+            const size_t per_block = (ctx->cap + (size_t)grid * kBlock - 1) / ((size_t)grid * kBlock) * kBlock;
+            M.ol_cap = (uint32_t)std::min<size_t>((grid + kMStripes - 1) / kMStripes * per_block, 0xFFFFFFFFu);
+            const size_t lists = (size_t)kLayerMaxA * kMStripes;
+            const size_t lb = (lists * M.ol_cap * 4 + 255) & ~(size_t)255;
+            const size_t cb = lists * (size_t)maxlen * 4;
+            if (lb + cb > ctx->mlayer_q_bytes) {
+                dev_free(ctx->mlayer_q);
+                ctx->mlayer_q_bytes = 0;
+                CHK(dev_alloc(ctx, &ctx->mlayer_q, lb + cb));
+                ctx->mlayer_q_bytes = lb + cb;
+            }
+            M.ol = (uint32_t*)ctx->mlayer_q;
+            M.ol_cnt = (uint32_t*)((char*)ctx->mlayer_q + lb);
+            HIPCHK(hipMemsetAsync(M.ol_cnt, 0, cb, ctx->stream));
+        }
+        const size_t n_ep = 2 + (size_t)maxlen + maxlen / every + 1;
+        CHK(eng.ensure_epochs(n_ep));
+        CHK(eng.ensure_counts(ctx->cap / kTile + 2));
+        const Epochs ep = eng.epochs();
+        HIPCHK(hipMemsetAsync(ctx->epochs, 0, Eng::epoch_words(ctx->epochs_cap) * sizeof(uint32_t), ctx->stream));
+        HIPCHK(hipMemsetAsync(mb + o_work, 0, (size_t)kStripes * 16 * 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(mb + o_out, 0, 2 * ne * 8, ctx->stream));
+        HIPCHK(hipMemsetAsync(mb + o_cnt, 0, ne * 4, ctx->stream));
+        const uint32_t init[2] = {(uint32_t)nt, 0u};
+        HIPCHK(hipMemcpyAsync(ep.n, &init[0], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ep.dead, &init[1], sizeof(uint32_t), hipMemcpyHostToDevice, ctx->stream));
+        const unsigned g_cap = grid_for(ctx->cap, kTile, 4096);
+        const unsigned g_idx = grid_for(std::max(ctx->cap, ctx->hslots), kBlock, 2048);
+        e = 1;
+        n_compact = 0;
+        for (uint32_t s = 0; s < maxlen; ++s) {
+            if (++ctx->layer_stamp >= (1u << 24)) {
+                HIPCHK(hipMemsetAsync(ctx->claim, 0, ctx->hslots * sizeof(uint64_t), ctx->stream));
+                ctx->layer_stamp = 1;
+            }
+            M.stamp = ctx->layer_stamp;
+            M.step = s;
+            hipLaunchKernelGGL((k_mlayer_claim<Alg, CP, K>), dim3(grid), dim3(kBlock), 0, ctx->stream, eng.store(),
+                               (const store_t*)ctx->coef, (store_t*)ctx->coef2, M, ep, e);
+            CHK(eng.launch_check());
+            hipLaunchKernelGGL((k_mlayer_replay<Alg, CP, K>), dim3(grid), dim3(kBlock), 0, ctx->stream, eng.store(),
+                               (const store_t*)ctx->coef, (store_t*)ctx->coef2, M, ep, e);
+            CHK(eng.launch_check());
+            std::swap(ctx->coef, ctx->coef2);
+            ++e;
+            if ((s + 1) % every == 0 && s + 1 < maxlen) {
+                const uint32_t ep_prev = e - 1;
+                hipLaunchKernelGGL((k_cmp_count<CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream, eng.store(), ep,
+                                   ep_prev, ctx->counts);
+                CHK(eng.launch_check());
+                hipLaunchKernelGGL(k_cmp_scan, dim3(1), dim3(kBlock), 0, ctx->stream, ep, ep_prev, ctx->counts);
+                CHK(eng.launch_check());
+                hipLaunchKernelGGL((k_cmp_scatter<NP, CP, K>), dim3(g_cap), dim3(kBlock), 0, ctx->stream,
+                                   eng.store(), ctx->key2, (store_t*)ctx->coef2, ep, ep_prev, e, ctx->counts);
+                CHK(eng.launch_check());
+                hipLaunchKernelGGL((k_cmp_copyback<NP, CP, K>), dim3(g_idx), dim3(kBlock), 0, ctx->stream,
+                                   eng.store(), ctx->key2, (const store_t*)ctx->coef2, ep, ep_prev, e);
+                CHK(eng.launch_check());
+                hipLaunchKernelGGL(k_index_build_async<NP>, dim3(g_idx), dim3(kBlock), 0, ctx->stream, ctx->key,
+                                   ctx->cap, ctx->index, ctx->hslots - 1, ep, ep_prev, e, 1, &ctx->ctrl->index_full);
+                CHK(eng.launch_check());
+                ++e;
+                ++n_compact;
+            }
+        }
+        hn.assign(e, 0);
+        hd.assign(e, 0);
+        ha.assign(e, 0);
+        hdd.assign(e, 0);
+        uint32_t hf[2] = {0, 0};
+        HIPCHK(hipMemcpyAsync(hn.data(), ep.n, e * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hd.data(), ep.dead, e * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ha.data(), ep.app, e * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hdd.data(), ep.dd, e * 4, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(hf, ep.flags, 8, hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipMemcpyAsync(ctx->ctrl_host, ctx->ctrl, sizeof(Ctrl), hipMemcpyDeviceToHost, ctx->stream));
+        HIPCHK(hipStreamSynchronize(ctx->stream));
+        if (ctx->ctrl_host->index_full)
+            return set_err(ctx, FFX_EHIP, "internal error: hash index full or layer degree bound broken (flags " +
+                                              std::to_string(ctx->ctrl_host->index_full) + ")");
+        if (!hf[0]) break;
+        want = 2 * ctx->cap;  // an append did not fit: a larger store, from the start
+    }
+    // slots after the last launch, as the next launch would see them
+    const uint32_t n_fin = hn[e - 1] + ha[e - 1];
+    int64_t peak = 0;
+    for (uint32_t q = 1; q < e; ++q)
+        peak = std::max<int64_t>(peak, (int64_t)(hn[q] + ha[q]) - (int64_t)((int32_t)hd[q] + hdd[q]));
+    if (n_fin > 0) {
+        hipLaunchKernelGGL((k_mexpect<Alg, CP, K>), dim3(grid_for(n_fin, kBlock, 4096)), dim3(kBlock), 0, ctx->stream,
+                           eng.store(), n_fin, (const uint64_t*)(mb + o_state), idshift, idmask, (double*)(mb + o_out),
+                           (unsigned int*)(mb + o_cnt));
+        CHK(eng.launch_check());
+    }
+    std::vector<double> hout(2 * ne);
+    std::vector<uint32_t> hcnt(ne);
+    std::vector<unsigned long long> hwork((size_t)kStripes * 16);
+    HIPCHK(hipMemcpyAsync(hout.data(), mb + o_out, 2 * ne * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hcnt.data(), mb + o_cnt, ne * 4, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipMemcpyAsync(hwork.data(), mb + o_work, hwork.size() * 8, hipMemcpyDeviceToHost, ctx->stream));
+    HIPCHK(hipStreamSynchronize(ctx->stream));
+    for (size_t b = 0; b < ne; ++b) {
+        out_expect[2 * elig[b]] = hout[2 * b];
+        out_expect[2 * elig[b] + 1] = hout[2 * b + 1];
+        if (out_terms) out_terms[elig[b]] = (int64_t)hcnt[b];
+    }
+    for (int c = 0; c < kStripes; ++c) {
+        ctx->prof.updates += (int64_t)hwork[(size_t)c * 16];
+        ctx->prof.outs += (int64_t)hwork[(size_t)c * 16 + 1];
+    }
+    ctx->prof.n_gates += n_rot;
+    ctx->prof.peak_terms = std::max(ctx->prof.peak_terms, peak);
+    ctx->prof.run_launches += maxlen;
+    ctx->prof.compactions += n_compact;
+    ctx->last_kind = 0;  // the store holds every instance's terms: no device-resident result of one call
+    ctx->mlayer_cap_hint = ctx->cap;
+    ran = true;
+    return FFX_OK;
+}
+
 template <int W, class CP, int K>
 int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, const ffx_options& o, int64_t slots,
                     double* out_expect, int64_t* out_terms, int32_t* out_fallback, const BatchConf* conf = nullptr) {
@@ -2196,18 +2606,29 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     for (int32_t i = 0; i < n_inst; ++i) max_terms = std::max(max_terms, inst[i].n_terms);
     const size_t S = pow2ceil((size_t)std::max<int64_t>(std::max<int64_t>(cf.slots, 2 * max_terms), 1024));
     if (S > 0x40000000ull) return set_err(ctx, FFX_ECAPACITY, "slot budget too large");
-    // schedules: eligible instances get their windows, the others the device-wide engine
+    // layer groups in the merged layout (ffx_mlayer.h) need every live term's degree
+    // <= kLayerMaxA: a degree limit that low and observables within it
+    bool try_ml = cf.merged == 1 && getenv("FFX_NO_MLAYER") == nullptr && maxdeg <= kLayerMaxA;
+    for (int32_t i = 0; i < n_inst && try_ml; ++i)
+        for (int64_t t = 0; t < inst[i].n_terms && try_ml; ++t) {
+            int d = 0;
+            for (int w = 0; w < W; ++w) d += __builtin_popcountll(inst[i].x[t * W + w] | inst[i].z[t * W + w]);
+            try_ml = d <= kLayerMaxA;
+        }
+    // schedules: eligible instances get their layer groups or windows, the others the
+    // device-wide engine
     std::vector<std::vector<BStep<Alg, CP>>> sched(n_inst);
-    std::vector<char> ok(n_inst, 0), done(n_inst, 0);
+    std::vector<MInstPlan<W, CP>> plans(try_ml ? n_inst : 0);
+    std::vector<char> ok(n_inst, 0), done(n_inst, 0), have_windows(n_inst, 0);
     std::vector<double> est(n_inst, 0.0);
-    auto build = [&](int32_t i) {
-        bool good = o.topk <= 0 && (size_t)inst[i].n_terms <= S / 2;
-        for (int64_t g = 0; g < inst[i].n_gates && good; ++g) good = inst[i].gates[g].kind == FFX_ROT;
-        if (!good) return;
+    std::vector<std::array<uint64_t, W>> used_of(n_inst);
+    auto rotation_steps = [&](int32_t i, std::vector<AStep>& sd) {
         const std::vector<Step> steps = pauli_schedule<W>(inst[i].gates, inst[i].n_gates, o.batched != 0);
-        std::vector<AStep> sd;
         std::vector<CliffGate> cl;
         pauli_async_steps<W, CP, K>(inst[i].gates, steps, o, sd, cl);
+    };
+    auto make_windows = [&](int32_t i, const std::vector<AStep>& sd) {
+        sched[i].clear();
         for (size_t t = 0; t < sd.size();) {
             const size_t kw = Eng::window_len(sd, t, cf.window);
             BStep<Alg, CP> b{};
@@ -2215,27 +2636,44 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             sched[i].push_back(b);
             t += kw;
         }
+        have_windows[i] = 1;
+    };
+    auto build = [&](int32_t i) {
+        bool good = o.topk <= 0 && (size_t)inst[i].n_terms <= S / 2;
+        for (int64_t g = 0; g < inst[i].n_gates && good; ++g) good = inst[i].gates[g].kind == FFX_ROT;
+        if (!good) return;
+        std::vector<AStep> sd;
+        rotation_steps(i, sd);
+        if (!try_ml || !plan_mgroups<W, CP, K>(sd, plans[i])) make_windows(i, sd);
         // work estimate: a circuit's live terms grow with its depth, about
         // rotations / qubits, so its work grows as rotations^2 / qubits (on the
         // 512 kicked-Ising instances within about +-25% of the counted updates)
-        uint64_t used[FFX_MAX_WORDS] = {0, 0, 0, 0};
+        uint64_t used[W] = {};
         for (int64_t g = 0; g < inst[i].n_gates; ++g)
             for (int w = 0; w < W; ++w) used[w] |= inst[i].gates[g].x[w] | inst[i].gates[g].z[w];
         int nq = 0;
         for (int w = 0; w < W; ++w) nq += __builtin_popcountll(used[w]);
+        for (int64_t t = 0; t < inst[i].n_terms; ++t)
+            for (int w = 0; w < W; ++w) used[w] |= inst[i].x[t * W + w] | inst[i].z[t * W + w];
+        for (int w = 0; w < W; ++w) used_of[i][w] = used[w];
         est[i] = (double)sd.size() * (double)sd.size() / (double)std::max(1, nq);
         ok[i] = 1;
     };
-    {
-        // schedules of the instances are independent: build them on host threads
+    // the instances are independent: their schedules are built on host threads
+    auto parallel = [&](const std::vector<int32_t>& which, auto&& fn) {
         const int nth = (int)std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-        std::atomic<int32_t> next{0};
+        std::atomic<size_t> next{0};
         std::vector<std::thread> pool;
-        for (int t = 0; t < std::min<int32_t>(nth, n_inst); ++t)
+        for (int t = 0; t < std::min<int>(nth, (int)which.size()); ++t)
             pool.emplace_back([&]() {
-                for (int32_t i = next++; i < n_inst; i = next++) build(i);
+                for (size_t q = next++; q < which.size(); q = next++) fn(which[q]);
             });
         for (auto& th : pool) th.join();
+    };
+    {
+        std::vector<int32_t> all((size_t)n_inst);
+        for (int32_t i = 0; i < n_inst; ++i) all[(size_t)i] = i;
+        parallel(all, build);
     }
     // largest estimated work first: workgroups are dispatched in launch order
     std::vector<int32_t> elig;
@@ -2243,11 +2681,29 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
         if (ok[i]) elig.push_back(i);
     std::stable_sort(elig.begin(), elig.end(), [&](int32_t a, int32_t b) { return est[a] > est[b]; });
     bool merged_ran = false;
-    if (cf.merged == 1 && !elig.empty()) {
-        CHK((run_pauli_merged<W, CP, K>(ctx, inst, elig, sched, out_expect, out_terms, merged_ran)));
-        if (merged_ran)
-            for (int32_t i : elig) done[i] = 1;
+    bool all_grouped = try_ml && !elig.empty();
+    for (int32_t i : elig) all_grouped = all_grouped && !have_windows[i];
+    if (all_grouped) {
+        uint64_t used[W] = {};
+        for (int32_t i : elig)
+            for (int w = 0; w < W; ++w) used[w] |= used_of[i][w];
+        CHK((run_pauli_mlayer<W, CP, K>(ctx, inst, elig, plans, used, o, out_expect, out_terms, merged_ran)));
     }
+    if (!merged_ran) {
+        // windows of the instances that only have layer groups so far
+        std::vector<int32_t> need;
+        for (int32_t i : elig)
+            if (!have_windows[i]) need.push_back(i);
+        parallel(need, [&](int32_t i) {
+            std::vector<AStep> sd;
+            rotation_steps(i, sd);
+            make_windows(i, sd);
+        });
+    }
+    if (!merged_ran && cf.merged == 1 && !elig.empty())
+        CHK((run_pauli_merged<W, CP, K>(ctx, inst, elig, sched, out_expect, out_terms, merged_ran)));
+    if (merged_ran)
+        for (int32_t i : elig) done[i] = 1;
     const size_t ne = merged_ran ? 0 : elig.size();
     int32_t fallback = n_inst - (int32_t)elig.size();
     if (ne > 0) {

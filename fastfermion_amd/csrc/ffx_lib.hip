@@ -69,12 +69,14 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     dev_free(ctx->topk_buf);
     dev_free(ctx->batch_pool);
     dev_free(ctx->merged_buf);
+    dev_free(ctx->mlayer_q);
     dev_free(ctx->crot);
     dev_free(ctx->claim);
     dev_free(ctx->layer_buf);
     ffx_mlp_free(ctx);
     if (ctx->batch_exec) (void)hipGraphExecDestroy(ctx->batch_exec);
     if (ctx->batch_host) (void)hipHostFree(ctx->batch_host);
+    if (ctx->mlayer_host) (void)hipHostFree(ctx->mlayer_host);
     for (hipEvent_t e : ctx->kev) (void)hipEventDestroy(e);
     if (ctx->ev0) (void)hipEventDestroy(ctx->ev0);
     if (ctx->ev1) (void)hipEventDestroy(ctx->ev1);

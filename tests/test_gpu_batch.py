@@ -1,7 +1,8 @@
-"""Batched propagation (ffx_pauli_propagate_batch) in both layouts -- merged (all
-instances in one term store, instance numbers in the top key bits, one launch
-per window step) and one workgroup per instance -- against the oracle run
-instance by instance, and against the device-wide engine.  The batch runs each
+"""Batched propagation (ffx_pauli_propagate_batch) in its layouts -- merged (all
+instances in one term store, instance numbers in the top key bits) with one
+launch per layer group step ("groups") or per window step ("windows"), and one
+workgroup per instance -- against the oracle run instance by instance, and
+against the device-wide engine.  The batch runs each
 instance's reference schedule and truncation, so FP64 expectations agree with
 the oracle to rounding of the final sum and the live term counts and work
 counters are identical; instances that outgrow their slot budget (one-workgroup
@@ -26,9 +27,11 @@ def ctx():
     c.close()
 
 
-@pytest.fixture(params=["merged", "workgroup"])
+@pytest.fixture(params=["groups", "windows", "workgroup"])
 def layout(request, monkeypatch):
-    monkeypatch.setenv("FFX_BATCH_MERGED", "1" if request.param == "merged" else "0")
+    monkeypatch.setenv("FFX_BATCH_MERGED", "0" if request.param == "workgroup" else "1")
+    if request.param == "windows":
+        monkeypatch.setenv("FFX_NO_MLAYER", "1")
     return request.param
 
 
@@ -87,6 +90,73 @@ def test_batch_uneven_two_words_and_states(ctx, layout):
     for i, (e, n, u) in enumerate(refs):
         assert abs(ex[i] - e) < 1e-12
         assert nt[i] == n
+
+
+def structured_circuit(rng, nq, nblocks):
+    """Blocks of rotations that the merged layout cuts into layer groups of every
+    kind: rotations on disjoint random supports (some qubits left out), runs of
+    Z-type or X-type strings (pairwise commuting, with repeated axes that end a
+    group), and a few unstructured rotations."""
+    recs = []
+    for _ in range(nblocks):
+        kind = int(rng.integers(0, 4))
+        if kind == 0:
+            perm = rng.permutation(nq).tolist()
+            i = 0
+            while i < nq:
+                w = int(rng.integers(1, 4))
+                qs = sorted(perm[i:i + w])
+                i += w
+                if rng.random() < 0.8:
+                    recs.append(("".join(rng.choice(list("XYZ"), len(qs))), tuple(qs), float(rng.uniform(-3, 3))))
+        elif kind in (1, 2):
+            letter = "Z" if kind == 1 else "X"
+            for _ in range(int(rng.integers(2, 14))):
+                w = int(rng.integers(1, 4))
+                qs = sorted(rng.choice(nq, w, replace=False).tolist())
+                recs.append((letter * w, tuple(qs), float(rng.uniform(-3, 3))))
+        else:
+            for _ in range(int(rng.integers(1, 4))):
+                w = int(rng.integers(1, 4))
+                qs = sorted(rng.choice(nq, w, replace=False).tolist())
+                recs.append(("".join(rng.choice(list("XYZ"), w)), tuple(qs), float(rng.uniform(-3, 3))))
+    return wl._pack(recs)
+
+
+@pytest.mark.parametrize("W,nq_hi", [(1, 40), (2, 120), (3, 180), (4, 240)])
+@pytest.mark.parametrize("kw", [
+    dict(maxdegree=8, mincoeff=1e-3, batched=False),
+    dict(maxdegree=6, mincoeff=1e-4, batched=True),
+    dict(maxdegree=8, mincoeff=1e-3, batched=False, max_xweight=2, xtrunc_period=2),
+])
+def test_batch_groups_structured_circuits(ctx, monkeypatch, W, nq_hi, kw):
+    """The merged layout in layer groups on circuits with disjoint, commuting and
+    unstructured runs, 1-4 key words, complex observables, both schedules and the
+    X/Y cap: every instance's <O> and live term count equal the oracle's, and the
+    batch's update counter equals the sum of the oracle's."""
+    monkeypatch.setenv("FFX_BATCH_MERGED", "1")
+    rng = np.random.default_rng(1000 * W + len(kw))
+    items, refs, upd = [], [], 0
+    for i in range(20):
+        nq = int(rng.integers(max(4, nq_hi // 3), nq_hi + 1))
+        g = structured_circuit(rng, nq, int(rng.integers(3, 12)))
+        w = int(rng.integers(1, 4))
+        qs = sorted(rng.choice(nq, w, replace=False).tolist())
+        ops = "".join(rng.choice(list("XYZ"), w))
+        coeff = 1.0 if i % 3 else 0.6 + 0.8j
+        x, z, c = wl.single_pauli(ops, qs, nq, coeff)
+        x, z = pad(x, W), pad(z, W)
+        st = [int(v) for v in rng.integers(0, 2 ** 62, W)]
+        items.append((g, x, z, c, st))
+        r = pyffo.pauli_propagate(g, W, x, z, c, **kw)
+        refs.append((pyffo.pauli_expect(r, st), len(r.c)))
+        upd += r.updates
+    ex, nt, fb = ctx.pauli_propagate_batch(items, W, **kw)
+    assert fb == 0
+    for i, (e, n) in enumerate(refs):
+        assert abs(ex[i] - e) < 1e-12, (i, ex[i], e)
+        assert nt[i] == n, (i, nt[i], n)
+    assert ctx.profile()["updates"] == upd
 
 
 def test_batch_slot_budget_fallback(ctx, monkeypatch):

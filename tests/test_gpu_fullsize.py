@@ -6,9 +6,10 @@
   C3  the same circuit in FP32 and FP32 compensated: |d<O>|, relative coefficient
       error on common keys and the key-set symmetric difference against FP64.
   C4  512 uneven kicked-Ising instances (L 6..12, so up to 144 qubits and 3 key
-      words; depth 4..24) batched in both layouts (one term store for all, one
-      workgroup each), FP64 and FP32 compensated: every instance's <O> and live
-      term count against the oracle run per instance.
+      words; depth 4..24) batched in every layout (one term store for all in
+      layer-group or window steps, one workgroup each), FP64 and FP32 compensated:
+      every instance's <O> and live term count against the oracle run per
+      instance.
   C5  the 6x6 Fermi-Hubbard Majorana circuit (20 Trotter steps, 6960 rotations) in
       both term orders, bit-identical to the serial Majorana oracle (src/majorana_propagation.h:18-33);
       and the learned pruner on the 30-layer kicked-Ising and the Fermi-Hubbard
@@ -123,12 +124,14 @@ def c4_oracle(c4):
     return np.array([r[0] for r in res]), np.array([r[1] for r in res])
 
 
-@pytest.mark.parametrize("layout", ["merged", "workgroup"])
+@pytest.mark.parametrize("layout", ["groups", "windows", "workgroup"])
 @pytest.mark.parametrize("prec,tol", [(_abi.FP64, 1e-13), (_abi.FP32C, 1e-12)])
 def test_c4_all_512_instances(ctx, c4, c4_oracle, monkeypatch, prec, tol, layout):
-    """Both layouts of the batch: one term store for all instances, and one
-    workgroup per instance."""
-    monkeypatch.setenv("FFX_BATCH_MERGED", "1" if layout == "merged" else "0")
+    """Every layout of the batch: one term store for all instances in layer-group
+    steps or in window steps, and one workgroup per instance."""
+    monkeypatch.setenv("FFX_BATCH_MERGED", "0" if layout == "workgroup" else "1")
+    if layout == "windows":
+        monkeypatch.setenv("FFX_NO_MLAYER", "1")
     inst, Wb = c4
 
     def pad(a):
