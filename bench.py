@@ -56,7 +56,42 @@ def parse():
                          "maxdegree 8, mincoeff 1e-4, FP64 (no CPU leg, no side measurements)")
     ap.add_argument("--fh-order", default="layers", choices=["layers", "bonds"],
                     help="Trotter term order of the Fermi-Hubbard circuit of --majorana")
+    ap.add_argument("--batch", action="store_true",
+                    help="run only the 512-instance batch (FP32 compensated, merged layer-group layout) --warmup "
+                         "then --steps times and print one JSON object (for profiling the batch kernels)")
     return ap.parse_args()
+
+
+def batch_only(a):
+    """The 512 uneven kicked-Ising instances, batched in the merged layer-group
+    layout, FP32 compensated: warm-up calls, then timed calls (for profiling)."""
+    import numpy as np
+
+    from fastfermion_amd import _abi, workloads as wl
+    from fastfermion_amd._native import Context
+
+    inst = wl.batch_kicked_ising(512)
+    Wb = max(_abi.words_for_bits(L * L) for (_, _, _, _, L, _) in inst)
+
+    def pad(v):
+        o = np.zeros((v.shape[0], Wb), np.uint64)
+        o[:, : v.shape[1]] = v
+        return o
+
+    items = [(g, pad(xi), pad(zi), ci, None) for (g, xi, zi, ci, L, d) in inst]
+    kw = dict(maxdegree=8, mincoeff=1e-4, batched=False, precision=_abi.FP32C)
+    os.environ["FFX_BATCH_MERGED"] = "1"
+    ctx = Context(0)
+    for _ in range(a.warmup):
+        ctx.pauli_propagate_batch(items, Wb, **kw)
+    t0 = time.perf_counter()
+    for _ in range(a.steps):
+        ex, nt, fb = ctx.pauli_propagate_batch(items, Wb, **kw)
+    dt = (time.perf_counter() - t0) / max(a.steps, 1)
+    p = ctx.profile()
+    print(json.dumps({"batch_only": {"instances": len(items), "ms_per_batch": 1e3 * dt, "updates": p["updates"],
+                                     "group_steps": p["run_launches"], "fallback_instances": fb}}), flush=True)
+    ctx.close()
 
 
 def cpu_baseline(a, gates, W, x, z, c, L, gpu_value):
@@ -527,6 +562,9 @@ def main():
     _GEO.update(rowcol=wl.lattice_geometry(L), obs=center, rpl=len(gates) // layers, L=L)
     kw = dict(maxdegree=8, mincoeff=1e-5, batched=bool(a.batched), precision=prec)
 
+    if a.batch:
+        batch_only(a)
+        return
     ctx = Context(0)
     if a.learned:
         print(json.dumps({"learned_pruner": learned_measurements(ctx, gates, W, x, z, c, None)}), flush=True)
