@@ -324,17 +324,20 @@ __global__ void __launch_bounds__(kBlock) k_cliff_async(Store<CP, K> st, const C
 }
 
 // ---- top-k truncation (src/pauli/truncate.h:21-39), device-side ----
-// Keeps the topk largest |c| of the live terms: a radix select over the bits of
-// |c| (a non-negative double orders like its bit pattern) finds V, the
-// (topk+1)-th largest magnitude, 8 bits per pass from the top; then every live
-// term below V is erased, and of the terms equal to V only as many as needed to
-// keep exactly topk (the reference erases the surplus ties in map order, which
-// is not reproducible; here the survivors among ties are arbitrary).
+// Keeps the topk largest |c| of the live terms.  A radix select over the bits of
+// |c| (a non-negative double orders like its bit pattern) finds V, the magnitude
+// of the topk-th largest term, 8 bits per pass from the top.  The reference
+// erases the surplus terms at V in its map's iteration order, which is not
+// reproducible; here the ties are ranked by a 64-bit hash of the key, with 8
+// more passes over the terms at V selecting H, the hash of the last kept tie, so
+// the survivors do not depend on the store's slot order or the grid.  Terms at
+// (V, H) -- distinct keys with equal hashes, practically never -- are kept in
+// slot order.  Then every live term below (V, H) is erased.
 struct TopK {
     uint32_t* hist;              // 256 bins of the current digit
     unsigned long long* prefix;  // magnitude bits selected so far
-    uint32_t* want;              // rank of the target among the terms matching the prefix
-    uint32_t* tiec;              // tie tickets
+    uint32_t* want;              // rank of the target among the terms matching the prefixes
+    unsigned long long* prefix2; // key-hash bits selected so far among the terms at V
 };
 
 template <class CP, int K>
@@ -350,9 +353,12 @@ __device__ __forceinline__ void state_after(const Epochs& ep, uint32_t e_prev, u
     live = n - (uint32_t)((int32_t)ep.dead[e_prev] + ep.dd[e_prev]);
 }
 
-template <class CP, int K>
+// Histogram of the digit at `shift` of the magnitude (phase 0) or, among the
+// terms at V, of the key hash (phase 1), over the terms matching the selected
+// higher digits.
+template <int NP, class CP, int K>
 __global__ void __launch_bounds__(kBlock) k_topk_hist(Store<CP, K> st, Epochs ep, uint32_t e_prev, TopK tk,
-                                                      uint32_t topk, int shift) {
+                                                      uint32_t topk, int shift, int phase) {
     if (run_stopped(ep)) return;
     uint32_t n, live;
     state_after(ep, e_prev, n, live);
@@ -361,19 +367,27 @@ __global__ void __launch_bounds__(kBlock) k_topk_hist(Store<CP, K> st, Epochs ep
     h[threadIdx.x] = 0;
     __syncthreads();
     const uint64_t mask = shift >= 56 ? 0ull : (~0ull << (shift + 8));
-    const uint64_t pre = shift >= 56 ? 0ull : (uint64_t)*tk.prefix;
+    const uint64_t V = (uint64_t)*tk.prefix;
+    const uint64_t pre = shift >= 56 ? 0ull : (uint64_t)(phase ? *tk.prefix2 : *tk.prefix);
     for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
         if (CP::dead(st.ld(st.coef, 0, i))) continue;
-        const uint64_t b = mag_bits<CP, K>(st, i);
+        uint64_t b = mag_bits<CP, K>(st, i);
+        if (phase) {
+            if (b != V) continue;
+            uint64_t k[NP];
+            key_load<NP>(st.key, i, k);
+            b = key_hash<NP>(k);
+        }
         if ((b & mask) == (pre & mask)) atomicAdd(&h[(b >> shift) & 255u], 1u);
     }
     __syncthreads();
     if (h[threadIdx.x]) atomicAdd(&tk.hist[threadIdx.x], h[threadIdx.x]);
 }
 
-// One block of 256 threads: choose the digit, update prefix and rank, clear the bins.
+// One block of 256 threads: choose the digit (largest first), update the prefix
+// and the rank, clear the bins.
 static __global__ void __launch_bounds__(kBlock) k_topk_pick(Epochs ep, uint32_t e_prev, TopK tk, uint32_t topk,
-                                                      int shift) {
+                                                      int shift, int phase) {
     if (run_stopped(ep)) return;
     uint32_t n, live;
     state_after(ep, e_prev, n, live);
@@ -382,8 +396,9 @@ static __global__ void __launch_bounds__(kBlock) k_topk_pick(Epochs ep, uint32_t
     h[threadIdx.x] = tk.hist[threadIdx.x];
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t want = shift >= 56 ? topk : *tk.want;
-        uint64_t pre = shift >= 56 ? 0ull : (uint64_t)*tk.prefix;
+        uint32_t want = (shift >= 56 && !phase) ? topk - 1 : *tk.want;
+        unsigned long long* pp = phase ? tk.prefix2 : tk.prefix;
+        uint64_t pre = shift >= 56 ? 0ull : (uint64_t)*pp;
         uint32_t cum = 0;
         int digit = 0;
         for (int b = 255; b >= 0; --b) {
@@ -393,47 +408,107 @@ static __global__ void __launch_bounds__(kBlock) k_topk_pick(Epochs ep, uint32_t
             }
             cum += h[b];
         }
-        *tk.prefix = (unsigned long long)(pre | ((uint64_t)digit << shift));
+        *pp = (unsigned long long)(pre | ((uint64_t)digit << shift));
         *tk.want = want - cum;
-        *tk.tiec = 0;
     }
     tk.hist[threadIdx.x] = 0;
 }
 
-// Epoch e: erase below the selected magnitude and the surplus ties, then (xw)
-// the X/Y-weight cap, which the reference applies after top-k (truncate.h:54-56).
+// Terms at (V, H) per tile of kTile slots (counts[tile]).
+template <int NP, class CP, int K>
+__global__ void __launch_bounds__(kBlock) k_topk_ties(Store<CP, K> st, Epochs ep, uint32_t e_prev, TopK tk,
+                                                      uint32_t topk, uint32_t* counts) {
+    if (run_stopped(ep)) return;
+    uint32_t n, live;
+    state_after(ep, e_prev, n, live);
+    if (live <= topk) return;
+    const uint64_t V = (uint64_t)*tk.prefix, H = (uint64_t)*tk.prefix2;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        int c = 0;
+#pragma unroll
+        for (int j = 0; j < kItems; ++j) {
+            const uint32_t i = t * kTile + j * kBlock + threadIdx.x;
+            if (i < n && !CP::dead(st.ld(st.coef, 0, i)) && mag_bits<CP, K>(st, i) == V) {
+                uint64_t k[NP];
+                key_load<NP>(st.key, i, k);
+                if (key_hash<NP>(k) == H) ++c;
+            }
+        }
+        c = block_sum_int(c);
+        if (threadIdx.x == 0) counts[t] = (uint32_t)c;
+    }
+}
+
+// One block: counts[tile] -> terms at (V, H) in the tiles before it.
+static __global__ void __launch_bounds__(kBlock) k_topk_tiescan(Epochs ep, uint32_t e_prev, uint32_t topk,
+                                                                 uint32_t* counts) {
+    if (run_stopped(ep)) return;
+    uint32_t n, live;
+    state_after(ep, e_prev, n, live);
+    if (live <= topk) return;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
+    uint32_t carry = 0;
+    for (uint32_t b0 = 0; b0 < tiles; b0 += kBlock) {
+        const uint32_t i = b0 + threadIdx.x;
+        const uint32_t v = i < tiles ? counts[i] : 0u;
+        uint32_t tot;
+        const uint32_t ex = block_exclusive_scan(v, &tot);
+        if (i < tiles) counts[i] = carry + ex;
+        carry += tot;
+        __syncthreads();
+    }
+}
+
+// Epoch e: erase the live terms below (V, H) and those at (V, H) past the first
+// `want` + 1 in slot order (ranked from the per-tile counts), then (xw) the
+// X/Y-weight cap, which the reference applies after top-k (truncate.h:54-56).
 template <class Alg, class CP, int K>
 __global__ void __launch_bounds__(kBlock) k_topk_apply(Store<CP, K> st, Epochs ep, uint32_t e, TopK tk,
-                                                       uint32_t topk, int xw, int max_xw) {
+                                                       uint32_t topk, int xw, int max_xw, const uint32_t* counts) {
     constexpr int NP = Alg::NP;
     uint32_t n, dead;
     if (!step_enter(ep, e, n, dead)) return;
     const bool sel = n - dead > topk;
     if (!sel && !xw) return;
-    const uint64_t V = sel ? (uint64_t)*tk.prefix : 0ull;
-    const uint32_t keep_ties = sel ? *tk.want : 0u;
-    const size_t cap = st.cap;
+    const uint64_t V = sel ? (uint64_t)*tk.prefix : 0ull, H = sel ? (uint64_t)*tk.prefix2 : 0ull;
+    const uint32_t keep_ties = sel ? *tk.want + 1u : 0u;
+    const uint32_t tiles = (n + kTile - 1) / kTile;
     int dd = 0;
-    for (uint32_t i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) {
-        if (CP::dead(st.ld(st.coef, 0, i))) continue;
-        bool kill = false;
-        if (sel) {
-            const uint64_t b = mag_bits<CP, K>(st, i);
-            if (b < V)
-                kill = true;
-            else if (b == V)
-                kill = atomicAdd(tk.tiec, 1u) >= keep_ties;
-        }
-        if (!kill && xw) {
-            uint64_t k[NP];
-            key_load<NP>(st.key, i, k);
-            kill = Alg::xweight(k) > max_xw;
-        }
-        if (kill) {
-            st.st(st.coef, 0, i, CP::make_dead());
+    for (uint32_t t = blockIdx.x; t < tiles; t += gridDim.x) {
+        uint32_t rank0 = sel ? counts[t] : 0u;  // terms at (V, H) in the slots before this tile's
 #pragma unroll
-            for (int kk = 1; kk < K; ++kk) st.st(st.coef, kk, i, CP::from_double(0.0));
-            ++dd;
+        for (int j = 0; j < kItems; ++j) {
+            const uint32_t i = t * kTile + j * kBlock + threadIdx.x;
+            const bool live = i < n && !CP::dead(st.ld(st.coef, 0, i));
+            bool kill = false, tie = false, have_key = false;
+            uint64_t k[NP];
+            if (live && sel) {
+                const uint64_t b = mag_bits<CP, K>(st, i);
+                if (b < V) {
+                    kill = true;
+                } else if (b == V) {
+                    key_load<NP>(st.key, i, k);
+                    have_key = true;
+                    const uint64_t hk = key_hash<NP>(k);
+                    kill = hk < H;
+                    tie = hk == H;
+                }
+            }
+            uint32_t tot;
+            const uint32_t r = rank0 + block_exclusive_scan(tie ? 1u : 0u, &tot);
+            rank0 += tot;
+            if (tie && r >= keep_ties) kill = true;
+            if (live && !kill && xw) {
+                if (!have_key) key_load<NP>(st.key, i, k);
+                kill = Alg::xweight(k) > max_xw;
+            }
+            if (kill) {
+                st.st(st.coef, 0, i, CP::make_dead());
+#pragma unroll
+                for (int kk = 1; kk < K; ++kk) st.st(st.coef, kk, i, CP::from_double(0.0));
+                ++dd;
+            }
         }
     }
     dd = block_sum_int(dd);

@@ -87,7 +87,7 @@ struct ffx_ctx {
     CliffGate* cliff = nullptr;
     size_t cliff_cap = 0;
     uint32_t* epochs = nullptr;
-    uint32_t* topk_buf = nullptr;  // top-k select scratch: 256 bins, prefix, rank, tie tickets
+    uint32_t* topk_buf = nullptr;  // top-k select scratch: 256 bins, magnitude prefix, rank, hash prefix
     void* batch_pool = nullptr;    // batched instances: term stores, schedules, states, inputs
     size_t batch_pool_bytes = 0;
     void* merged_buf = nullptr;    // merged batch: windows or groups, schedule offsets, states, results
@@ -1007,7 +1007,7 @@ struct Engine {
         tk.hist = ctx->topk_buf;
         tk.prefix = (unsigned long long*)(ctx->topk_buf + 256);
         tk.want = ctx->topk_buf + 258;
-        tk.tiec = ctx->topk_buf + 259;
+        tk.prefix2 = (unsigned long long*)(ctx->topk_buf + 260);
         return tk;
     }
 
@@ -1402,16 +1402,25 @@ struct Engine {
                     const uint32_t ep_prev = e - 1;
                     const TopK tk = topk_scratch();
                     const unsigned g_sel = grid_for(ctx->cap, 4 * kBlock, 1024);
-                    for (int shift = 56; shift >= 0; shift -= 8) {
-                        hipLaunchKernelGGL((k_topk_hist<CP, K>), dim3(g_sel), dim3(kBlock), 0, ctx->stream, store(), ep,
-                                           ep_prev, tk, topk, shift);
-                        CHK(launch_check());
-                        hipLaunchKernelGGL(k_topk_pick, dim3(1), dim3(kBlock), 0, ctx->stream, ep, ep_prev, tk, topk,
-                                           shift);
-                        CHK(launch_check());
-                    }
+                    // the magnitude of the topk-th largest term, then the key hash that ranks
+                    // the terms at that magnitude
+                    for (int phase = 0; phase < 2; ++phase)
+                        for (int shift = 56; shift >= 0; shift -= 8) {
+                            hipLaunchKernelGGL((k_topk_hist<NP, CP, K>), dim3(g_sel), dim3(kBlock), 0, ctx->stream,
+                                               store(), ep, ep_prev, tk, topk, shift, phase);
+                            CHK(launch_check());
+                            hipLaunchKernelGGL(k_topk_pick, dim3(1), dim3(kBlock), 0, ctx->stream, ep, ep_prev, tk,
+                                               topk, shift, phase);
+                            CHK(launch_check());
+                        }
+                    hipLaunchKernelGGL((k_topk_ties<NP, CP, K>), dim3(g_sel), dim3(kBlock), 0, ctx->stream, store(),
+                                       ep, ep_prev, tk, topk, ctx->counts);
+                    CHK(launch_check());
+                    hipLaunchKernelGGL(k_topk_tiescan, dim3(1), dim3(kBlock), 0, ctx->stream, ep, ep_prev, topk,
+                                       ctx->counts);
+                    CHK(launch_check());
                     hipLaunchKernelGGL((k_topk_apply<Alg, CP, K>), dim3(g_sel), dim3(kBlock), 0, ctx->stream, store(), ep,
-                                       e, tk, topk, steps[t + kw - 1].topk_xw, max_xw);
+                                       e, tk, topk, steps[t + kw - 1].topk_xw, max_xw, (const uint32_t*)ctx->counts);
                     CHK(launch_check());
                     step_of[e] = (int64_t)t;
                     ++e;

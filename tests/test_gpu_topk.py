@@ -2,8 +2,9 @@
 
 Top-k keeps the k largest |c| after the threshold and before the X/Y cap.  The
 reference erases surplus exact ties at the boundary in hash-map order, which
-no other implementation can reproduce; the device keeps an arbitrary subset
-of them.  So the checks are: the multiset of kept magnitudes is identical,
+no other implementation can reproduce; the device keeps the ties with the
+largest key hashes, the same set on every run and grid.  So the checks are:
+the multiset of kept magnitudes is identical,
 and every kept key strictly above the boundary magnitude is identical with a
 bit-identical coefficient.  Random axes and angles make boundary ties (and
 with them diverging later batches) unlikely; one case forces ties on purpose
@@ -17,6 +18,7 @@ import numpy as np
 import pytest
 
 import pyffo
+from _helpers import canon
 from fastfermion_amd import _abi, workloads as wl
 from fastfermion_amd._native import Context
 
@@ -108,6 +110,30 @@ def test_topk_with_forced_ties(ctx):
         dev = ctx.pauli_propagate(g, 1, x, z, c, topk=topk)
         assert len(dev.c) == min(topk, 256)
         assert np.array_equal(np.sort(np.abs(dev.c)), np.sort(np.abs(ref.c)))
+
+
+def test_topk_ties_deterministic(ctx, monkeypatch):
+    """Surplus ties at the boundary are ranked by a hash of the key: the kept set is
+    the same on every run and for every launch grid (the store's slot order is not:
+    blocks reserve their appends in whatever order they finish)."""
+    nq = 8
+    g = wl._pack([("X", (q,), 0.4) for q in range(nq)] + [("Y", (q,), 0.3) for q in range(nq)])
+    x, z, c = wl.single_pauli("ZZZZZZZZ", list(range(nq)), nq)
+    runs = []
+    for grid in (None, "7", "64", None, "1000"):
+        if grid is None:
+            monkeypatch.delenv("FFX_WIN_GRID", raising=False)
+        else:
+            monkeypatch.setenv("FFX_WIN_GRID", grid)
+        for topk in (5, 37, 100):
+            dev = ctx.pauli_propagate(g, 1, x, z, c, topk=topk, batched=False)
+            assert len(dev.c) == topk
+            k, v = canon(dev.x, dev.z, dev.c)
+            runs.append((topk, grid, k, v))
+    for topk in (5, 37, 100):
+        same = [(k, v) for (t, _, k, v) in runs if t == topk]
+        for k, v in same[1:]:
+            assert np.array_equal(k, same[0][0]) and np.array_equal(v, same[0][1]), topk
 
 
 def test_topk_precisions_and_api(ctx):
