@@ -210,7 +210,7 @@ __device__ __forceinline__ void dlayer_member(const uint64_t (&ck)[Alg::NP], uin
 }
 
 template <class Alg, class CP, int K>
-__global__ void __launch_bounds__(kBlock) k_dlayer_async(Store<CP, K> st, const typename CP::store_t* __restrict__ cin,
+__global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, const typename CP::store_t* __restrict__ cin,
                                                          typename CP::store_t* __restrict__ cout,
                                                          DLayer<Alg::W, CP, Alg> L, Epochs ep, uint32_t e) {
     constexpr int NP = Alg::NP;
