@@ -159,6 +159,26 @@ def test_batch_groups_structured_circuits(ctx, monkeypatch, W, nq_hi, kw):
     assert ctx.profile()["updates"] == upd
 
 
+def test_batch_instance_numbers_that_do_not_fit(ctx, monkeypatch):
+    """64 one-word instances on 60 qubits: 6 bits of instance number do not fit above
+    qubit 59, so the merged layouts step aside for the one-workgroup layout, with the
+    same results."""
+    monkeypatch.setenv("FFX_BATCH_MERGED", "1")
+    rng = np.random.default_rng(77)
+    items, refs = [], []
+    for i in range(64):
+        nq = 60
+        g = structured_circuit(rng, nq, 3)
+        q = int(rng.integers(0, nq))
+        x, z, c = wl.single_pauli("Z", [q], nq)
+        items.append((g, x, z, c, None))
+        r = pyffo.pauli_propagate(g, 1, x, z, c, maxdegree=6, mincoeff=1e-3, batched=False)
+        refs.append((pyffo.pauli_expect(r, [0]), len(r.c)))
+    ex, nt, fb = ctx.pauli_propagate_batch(items, 1, maxdegree=6, mincoeff=1e-3, batched=False)
+    for i, (e, n) in enumerate(refs):
+        assert abs(ex[i] - e) < 1e-12 and nt[i] == n, i
+
+
 def test_batch_slot_budget_fallback(ctx, monkeypatch):
     """A budget far below the instances' peaks: they stop in the batch launch
     and rerun on the device-wide engine with identical results (one-workgroup
