@@ -115,6 +115,7 @@ struct ffx_ctx {
         int block = 0, window = 0;
         int64_t slots = 0;
         int merged = 0;
+        int compact_every = 0, group_max = 0;
     };
     std::vector<BatchCfg> batch_tuned;
     void* batch_host = nullptr;  // pinned
@@ -1966,7 +1967,10 @@ struct BatchConf {
     int block = 0;
     int window = 0;
     int64_t slots = 0;
-    int merged = -1;  // 1: one term store for all instances (ffx_merged.h), 0: one workgroup each
+    int merged = -1;         // 1: one term store, layer-group steps (ffx_mlayer.h); 2: one term store,
+                             // window steps (ffx_merged.h); 0: one workgroup each (ffx_batch.h)
+    int compact_every = 0;   // layer-group steps between compaction checks
+    int group_max = 0;       // most commuting rotations with overlapping supports in a group
 };
 
 // Merged batch (ffx_merged.h): the instances elig[0..ne) in one term store, one
@@ -2196,10 +2200,12 @@ struct MInstPlan {
 };
 
 template <int W, class CP, int K>
-bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>& sd, MInstPlan<W, CP>& P) {
+bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>& sd, MInstPlan<W, CP>& P,
+                  int group_max) {
     using Eng = Engine<PauliAlg<W>, CP, K>;
     constexpr int NP = 2 * W;
-    size_t dmax = kMGroupMax, gmax = kLayerMaxA;
+    size_t dmax = kMGroupMax;
+    size_t gmax = group_max > 0 ? std::min<size_t>(kLayerMaxA, (size_t)group_max) : (size_t)kLayerMaxA;
     if (const char* ev = getenv("FFX_MGROUP_MAX"))
         if (atoi(ev) > 0) dmax = std::min<size_t>(kMGroupMax, (size_t)atoi(ev));
     if (const char* ev = getenv("FFX_MGENERAL_MAX"))
@@ -2293,14 +2299,15 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
 }
 
 // Merged batch in layer groups (ffx_mlayer.h): the instances elig[0..ne) in one
-// term store, one launch per group step of the longest schedule, the
-// device-decided compaction every FFX_MCOMPACT steps (default 4).  `ran` stays
+// term store, a claim and a replay launch per group step of the longest schedule,
+// the device-decided compaction every `compact_every` steps (default 4;
+// FFX_MCOMPACT overrides).  `ran` stays
 // false (nothing done) when the instance numbers do not fit above every qubit
 // the instances use (`used`: the union of their qubit masks).
 template <int W, class CP, int K>
 int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<int32_t>& elig,
                      const std::vector<MInstPlan<W, CP>>& plans, const uint64_t (&used)[W], const ffx_options& o,
-                     double* out_expect, int64_t* out_terms, bool& ran) {
+                     int compact_every, double* out_expect, int64_t* out_terms, bool& ran) {
     using Alg = PauliAlg<W>;
     using Eng = Engine<Alg, CP, K>;
     using store_t = typename CP::store_t;
@@ -2427,7 +2434,7 @@ int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
     M.maxdeg = degree_limit(o.maxdegree);
     M.max_xw = o.max_xweight;
     Eng eng(ctx);
-    uint32_t every = 4;
+    uint32_t every = compact_every > 0 ? (uint32_t)compact_every : 4u;
     if (const char* ev = getenv("FFX_MCOMPACT"))
         if (atoi(ev) > 0) every = (uint32_t)atoi(ev);
     if (ctx->n_cu <= 0) {
@@ -2593,16 +2600,19 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     const int maxdeg = degree_limit(o.maxdegree);
     // configuration: explicit > autotuned for this (W, precision, K) > defaults
     const int prec_id = std::is_same<CP, F64>::value ? FFX_FP64 : (std::is_same<CP, F32>::value ? FFX_FP32 : FFX_FP32C);
-    BatchConf cf{kBlockBatch, kWin, 32768, 1};
+    BatchConf cf{kBlockBatch, kWin, 32768, 1, 0, 0};
     for (const auto& t : ctx->batch_tuned)
-        if (t.W == W && t.prec == prec_id && t.K == K) cf = BatchConf{t.block, t.window, t.slots, t.merged};
+        if (t.W == W && t.prec == prec_id && t.K == K)
+            cf = BatchConf{t.block, t.window, t.slots, t.merged, t.compact_every, t.group_max};
     if (conf) {
         if (conf->block) cf.block = conf->block;
         if (conf->window) cf.window = conf->window;
         if (conf->slots) cf.slots = conf->slots;
         if (conf->merged >= 0) cf.merged = conf->merged;
+        if (conf->compact_every) cf.compact_every = conf->compact_every;
+        if (conf->group_max) cf.group_max = conf->group_max;
     }
-    if (const char* ev = getenv("FFX_BATCH_MERGED")) cf.merged = atoi(ev) != 0 ? 1 : 0;
+    if (const char* ev = getenv("FFX_BATCH_MERGED")) cf.merged = std::max(0, std::min(2, atoi(ev)));
     if (slots > 0) cf.slots = slots;
     if (const char* ev = getenv("FFX_BATCH_BLOCK")) {
         const int v = atoi(ev);
@@ -2618,6 +2628,7 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     // layer groups in the merged layout (ffx_mlayer.h) need every live term's degree
     // <= kLayerMaxA: a degree limit that low and observables within it
     bool try_ml = cf.merged == 1 && getenv("FFX_NO_MLAYER") == nullptr && maxdeg <= kLayerMaxA;
+    const bool merged_any = cf.merged == 1 || cf.merged == 2;
     for (int32_t i = 0; i < n_inst && try_ml; ++i)
         for (int64_t t = 0; t < inst[i].n_terms && try_ml; ++t) {
             int d = 0;
@@ -2653,7 +2664,7 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
         if (!good) return;
         std::vector<AStep> sd;
         rotation_steps(i, sd);
-        if (!try_ml || !plan_mgroups<W, CP, K>(sd, plans[i])) make_windows(i, sd);
+        if (!try_ml || !plan_mgroups<W, CP, K>(sd, plans[i], cf.group_max)) make_windows(i, sd);
         // work estimate: a circuit's live terms grow with its depth, about
         // rotations / qubits, so its work grows as rotations^2 / qubits (on the
         // 512 kicked-Ising instances within about +-25% of the counted updates)
@@ -2696,7 +2707,8 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
         uint64_t used[W] = {};
         for (int32_t i : elig)
             for (int w = 0; w < W; ++w) used[w] |= used_of[i][w];
-        CHK((run_pauli_mlayer<W, CP, K>(ctx, inst, elig, plans, used, o, out_expect, out_terms, merged_ran)));
+        CHK((run_pauli_mlayer<W, CP, K>(ctx, inst, elig, plans, used, o, cf.compact_every, out_expect, out_terms,
+                                        merged_ran)));
     }
     if (!merged_ran) {
         // windows of the instances that only have layer groups so far
@@ -2709,7 +2721,7 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
             make_windows(i, sd);
         });
     }
-    if (!merged_ran && cf.merged == 1 && !elig.empty())
+    if (!merged_ran && merged_any && !elig.empty())
         CHK((run_pauli_merged<W, CP, K>(ctx, inst, elig, sched, out_expect, out_terms, merged_ran)));
     if (merged_ran)
         for (int32_t i : elig) done[i] = 1;

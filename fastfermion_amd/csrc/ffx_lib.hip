@@ -390,15 +390,18 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     std::vector<ffx_instance> sub((size_t)m);
     for (int32_t i = 0; i < m; ++i) sub[(size_t)i] = inst[order[(size_t)i]];
     std::vector<double> ex(2 * (size_t)m);
-    const int blocks[3] = {128, 256, 512};
+    const int blocks[2] = {256, 512};
     const int windows[2] = {3, 4};
     const int64_t budgets[2] = {8192, 32768};
-    // candidates: the merged layout per window length, then the one-workgroup layout
+    // candidates: the layer-group layout per compaction interval and general group size,
+    // the window-step layout, then the one-workgroup layout
     std::vector<BatchConf> cands;
-    for (int wi : windows) cands.push_back(BatchConf{kBlockBatch, wi, 32768, 1});
+    for (int ce : {2, 4, 8})
+        for (int gm : {4, 6, 8}) cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 1, ce, gm});
+    cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 2, 0, 0});
     for (int bl : blocks)
         for (int wi : windows)
-            for (int64_t sl : budgets) cands.push_back(BatchConf{bl, wi, sl, 0});
+            for (int64_t sl : budgets) cands.push_back(BatchConf{bl, wi, sl, 0, 0, 0});
     double best_ms = 1e300;
     BatchConf best{};
     int32_t tried = 0;
@@ -426,6 +429,8 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     t.window = best.window;
     t.slots = best.slots;
     t.merged = best.merged;
+    t.compact_every = best.compact_every;
+    t.group_max = best.group_max;
     bool replaced = false;
     for (auto& e : ctx->batch_tuned)
         if (e.W == t.W && e.prec == t.prec && e.K == t.K) {
@@ -437,6 +442,8 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     out->window = best.window;
     out->slots = best.slots;
     out->merged = best.merged;
+    out->compact_every = best.compact_every;
+    out->group_max = best.group_max;
     out->reserved = 0;
     out->ms = best_ms;
     out->candidates = tried;

@@ -195,24 +195,30 @@ int ffx_pauli_propagate_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_
 
 /* Launch configuration of batched propagation chosen by ffx_batch_autotune. */
 typedef struct ffx_batch_tuning {
-    int32_t block;       /* threads of the workgroup that runs one instance (128, 256 or 512) */
-    int32_t window;      /* most commuting rotations fused into one window (1..4) */
-    int64_t slots;       /* term slots per instance */
-    double ms;           /* time of the sample batch with this configuration */
-    int32_t candidates;  /* configurations timed */
-    int32_t sample;      /* instances in the sample batch */
-    int32_t merged;      /* 1: merged layout (block and slots unused), 0: one workgroup per instance */
+    int32_t block;          /* threads of the workgroup that runs one instance (128, 256 or 512) */
+    int32_t window;         /* most commuting rotations fused into one window (1..4) */
+    int64_t slots;          /* term slots per instance */
+    double ms;              /* time of the sample batch with this configuration */
+    int32_t candidates;     /* configurations timed */
+    int32_t sample;         /* instances in the sample batch */
+    int32_t merged;         /* layout: 1 all instances in one term store, one launch pair per layer-group
+                               step; 2 the same store, one launch per window step; 0 one workgroup per
+                               instance (block and slots apply to 0 only) */
+    int32_t compact_every;  /* layout 1: group steps between device-decided compaction checks */
+    int32_t group_max;      /* layout 1: most commuting rotations with overlapping supports in a group (1..8) */
     int32_t reserved;
 } ffx_batch_tuning;
 
 /* Autotuner of batched propagation (no reference counterpart): times the
- * merged layout with windows of 3 and 4 rotations and every combination of
- * workgroup size, window length and slot budget of the one-workgroup layout on
- * the whole batch (sample <= 0) or on the `sample` instances with the longest circuits,
- * with the options of `opts`; prefers configurations without device-wide
- * fallbacks, and keeps the fastest on the context for this key width, precision
- * and coefficient kind; later ffx_pauli_propagate_batch calls with slots = 0
- * use it. */
+ * layer-group layout with compaction checks every 2, 4 or 8 steps and groups of
+ * up to 4, 6 or 8 overlapping commuting rotations, the window-step layout, and
+ * the one-workgroup layout with every combination of workgroup size (256, 512),
+ * window length (3, 4) and slot budget (8192, 32768), on the whole batch
+ * (sample <= 0) or on the `sample` instances with the longest circuits, with the
+ * options of `opts`; prefers configurations without device-wide fallbacks, and
+ * keeps the fastest on the context for this key width, precision and
+ * coefficient kind; later ffx_pauli_propagate_batch calls with slots = 0 use
+ * it. */
 int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, int32_t n_words,
                        const ffx_options* opts, int32_t sample, ffx_batch_tuning* out);
 

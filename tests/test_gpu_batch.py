@@ -282,17 +282,20 @@ def test_graph_replays_match_direct_launches(ctx, monkeypatch):
 
 
 def test_autotune_keeps_results(ctx):
-    """The autotuned launch (layout, workgroup size, window length, slot budget)
-    changes only speed: FP64 term counts stay identical to the default launch and
-    expectations agree to rounding of their sums."""
+    """The autotuned launch (layout, compaction interval, group size, workgroup
+    size, window length, slot budget) changes only speed: FP64 term counts stay
+    identical to the default launch and expectations agree to rounding of their
+    sums."""
     W = 1
     inst = instances(40, 31, l_lo=3, l_span=5, d_lo=3, d_span=8)
     items = [(g, pad(x, W), pad(z, W), c, None) for (g, x, z, c, L, d) in inst]
     kw = dict(maxdegree=8, mincoeff=1e-4, batched=False)
     want = ctx.pauli_propagate_batch(items, W, **kw)
     t = ctx.batch_autotune(items, W, sample=16, **kw)
-    assert t["block"] in (128, 256, 512) and 1 <= t["window"] <= 4 and t["slots"] >= 1024
-    assert t["merged"] in (0, 1)
-    assert t["candidates"] == 14 and t["sample"] == 16 and t["ms"] > 0
+    assert t["block"] in (256, 512) and 1 <= t["window"] <= 4 and t["slots"] >= 1024
+    assert t["merged"] in (0, 1, 2)
+    if t["merged"] == 1:
+        assert t["compact_every"] in (2, 4, 8) and t["group_max"] in (4, 6, 8)
+    assert t["candidates"] == 18 and t["sample"] == 16 and t["ms"] > 0
     got = ctx.pauli_propagate_batch(items, W, **kw)
     assert np.max(np.abs(got[0] - want[0])) < 1e-13 and np.array_equal(got[1], want[1])
