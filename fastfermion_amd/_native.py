@@ -83,6 +83,14 @@ class Instance(ctypes.Structure):
                 ("n_terms", ctypes.c_int64), ("basis_state", ctypes.c_uint64 * 4)]
 
 
+# numpy image of Instance (same field offsets), for filling many records at once
+_INSTANCE_DTYPE = np.dtype([("gates", "<u8"), ("n_gates", "<i8"), ("x", "<u8"), ("z", "<u8"), ("c_re", "<u8"),
+                            ("c_im", "<u8"), ("n_terms", "<i8"), ("basis_state", "<u8", (4,))])
+assert _INSTANCE_DTYPE.itemsize == ctypes.sizeof(Instance)
+for _f, _t in Instance._fields_:
+    assert _INSTANCE_DTYPE.fields[_f][1] == getattr(Instance, _f).offset
+
+
 class BatchTuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("window", ctypes.c_int32), ("slots", ctypes.c_int64),
                 ("ms", ctypes.c_double), ("candidates", ctypes.c_int32), ("sample", ctypes.c_int32),
@@ -313,22 +321,32 @@ class Context:
 
     @staticmethod
     def _instances(instances, n_words):
-        keep = []
-        arr = (Instance * max(len(instances), 1))()
+        """ffx_instance records of a batch, filled column-wise in a numpy array of the
+        same layout (a ctypes field assignment per record costs ~10 us); returns the
+        record pointer and the arrays it points into, which must outlive the call."""
+        n = len(instances)
+        rec = np.zeros(max(n, 1), dtype=_INSTANCE_DTYPE)
+        keep = [rec]
+        cols = {k: [0] * n for k in ("gates", "n_gates", "x", "z", "c_re", "c_im", "n_terms")}
         for i, (gates, x, z, c, state) in enumerate(instances):
             gates = np.ascontiguousarray(gates, dtype=PAULI_GATE_DTYPE)
             x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, n_words)
             z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, n_words)
             c = np.asarray(c, dtype=np.complex128).reshape(-1)
             re = np.ascontiguousarray(c.real)
-            im = np.ascontiguousarray(c.imag) if np.any(c.imag != 0) else None
+            im = np.ascontiguousarray(c.imag) if c.imag.any() else None
             keep += [gates, x, z, re, im]
-            arr[i].gates, arr[i].n_gates = _p(gates), len(gates)
-            arr[i].x, arr[i].z, arr[i].c_re, arr[i].c_im = _p(x), _p(z), _p(re), _p(im)
-            arr[i].n_terms = len(c)
-            for w, v in enumerate(list(state or [])[:4]):
-                arr[i].basis_state[w] = int(v)
-        return arr, keep
+            cols["gates"][i], cols["n_gates"][i] = gates.ctypes.data, len(gates)
+            cols["x"][i], cols["z"][i], cols["c_re"][i] = x.ctypes.data, z.ctypes.data, re.ctypes.data
+            cols["c_im"][i] = 0 if im is None else im.ctypes.data
+            cols["n_terms"][i] = len(c)
+            if state:
+                for w, v in enumerate(list(state)[:4]):
+                    rec["basis_state"][i, w] = int(v)
+        if n:
+            for k, v in cols.items():
+                rec[k][:n] = v
+        return rec.ctypes.data_as(ctypes.POINTER(Instance)), keep
 
     def majorana_propagate(self, gates, n_words, m, c, maxdegree=None, mincoeff=0.0, precision=FP64,
                            download=True, occ=None, want_expect=False):
