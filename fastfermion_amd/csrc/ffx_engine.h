@@ -21,6 +21,7 @@
 #include <cstring>
 #include <string>
 #include <type_traits>
+#include <unordered_map>
 #include <utility>
 #include <thread>
 #include <vector>
@@ -2191,12 +2192,15 @@ int run_pauli_merged(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
 // the longer of a run on pairwise disjoint supports (at most kMGroupMax) and a
 // run of pairwise commuting rotations with independent axes (at most
 // kLayerMaxA; a single rotation is one), the disjoint run on ties.  Offsets in
-// the records are local to the instance.  False if a step is not a rotation.
+// the records are local to the instance.  Groups with the same rotation records
+// (the same layer in every Trotter step) share one copy of the records and of
+// the position table.  False if a step is not a rotation.
 template <int W, class CP>
 struct MInstPlan {
     std::vector<MGroup<W>> groups;
     std::vector<DLayerGate<PauliAlg<W>, CP>> gates;
     std::vector<uint8_t> pos;
+    int64_t n_rot = 0;  // rotations of the schedule (records are shared, so gates.size() is fewer)
 };
 
 template <int W, class CP, int K>
@@ -2210,12 +2214,16 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
         if (atoi(ev) > 0) dmax = std::min<size_t>(kMGroupMax, (size_t)atoi(ev));
     if (const char* ev = getenv("FFX_MGENERAL_MAX"))
         if (atoi(ev) > 0) gmax = std::min<size_t>(kLayerMaxA, (size_t)atoi(ev));
+    using Gate = DLayerGate<PauliAlg<W>, CP>;
     P.groups.clear();
     P.gates.clear();
     P.pos.clear();
+    P.n_rot = (int64_t)sd.size();
     for (const auto& a : sd)
         if (a.kind != 0) return false;
-    P.gates.reserve(sd.size());
+    // first group seen with each hash of (records, kind); a hit with equal bytes
+    // takes that group's records instead of a new copy
+    std::unordered_map<uint64_t, uint32_t> seen;
     for (size_t t = 0; t < sd.size();) {
         // run on pairwise disjoint supports
         size_t kd = 0;
@@ -2292,6 +2300,32 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
         if (!general)
             for (int w = 0; w < W; ++w)
                 for (uint64_t b = G.umask[w]; b; b &= b - 1) P.pos.push_back(upos[64 * w + __builtin_ctzll(b)]);
+        {
+            // the position table is a function of the records and the kind, so equal
+            // records of the same kind share it too
+            const unsigned char* gb = (const unsigned char*)(P.gates.data() + G.gate_off);
+            const size_t nb = k * sizeof(Gate);
+            uint64_t h = 0x9E3779B97F4A7C15ull ^ (uint64_t)k ^ ((uint64_t)G.general << 32);
+            for (size_t q = 0; q + 8 <= nb; q += 8) {
+                uint64_t v;
+                std::memcpy(&v, gb + q, 8);
+                h = (h ^ v) * 0xFF51AFD7ED558CCDull;
+                h ^= h >> 29;
+            }
+            const auto it = seen.find(h);
+            if (it == seen.end()) {
+                seen.emplace(h, (uint32_t)P.groups.size());
+            } else {
+                const MGroup<W>& E = P.groups[it->second];
+                if (E.nr == G.nr && E.general == G.general &&
+                    std::memcmp(P.gates.data() + E.gate_off, gb, nb) == 0) {
+                    P.gates.resize(G.gate_off);
+                    P.pos.resize(G.pos_off);
+                    G.gate_off = E.gate_off;
+                    G.pos_off = E.pos_off;
+                }
+            }
+        }
         P.groups.push_back(G);
         t += k;
     }
@@ -2341,7 +2375,7 @@ int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
         npos += P.pos.size();
         nt += (size_t)inst[elig[b]].n_terms;
         maxlen = std::max(maxlen, glen[b]);
-        n_rot += (int64_t)P.gates.size();
+        n_rot += P.n_rot;
     }
     // device buffer: groups, gates, positions, gbase, glen, states | sums, counts, work stripes
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
