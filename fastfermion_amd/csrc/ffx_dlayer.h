@@ -37,6 +37,8 @@ namespace ffx {
 // appends staged in LDS per block iteration (more go straight to HBM); 256 keeps
 // the kernel at 4 blocks of 256 threads per CU for keys of up to 4 words
 constexpr int kDLayerStage = 256;
+// consecutive slots per run of a block iteration's tile
+constexpr int kDLayerRun = 16;
 
 template <class Alg, class CP>
 struct DLayerGate {
@@ -60,6 +62,7 @@ struct DLayer {
     int max_xw;
     uint32_t stamp;  // this launch's stamp (24 bits, nonzero)
     int general;     // 1: at most kLayerMaxA commuting rotations with overlapping supports
+    int spread;      // 1: small tiles taken in runs spread over the store (layers with threshold points)
 };
 
 // Units of a key (qubits in the Pauli support, Majorana indices), word w.
@@ -238,8 +241,21 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
             tile = t;
             break;
         }
-    const uint32_t step = gridDim.x * tile;
-    if (blockIdx.x * tile >= n) return;  // block-uniform
+    // layers with threshold points (L.spread), when every slot fits into one block
+    // iteration (tiles of at most 64 slots): the tile is taken in runs of kDLayerRun
+    // consecutive slots spread over the store (run j of block b at
+    // ((j * grid + b) * kDLayerRun)).  The fresh members of one orbit are appended
+    // side by side and their orbits under the next layer are often all large; with
+    // truncation inside the layer such an orbit leaves few members but costs its
+    // full size, and contiguous tiles would hand a few blocks most of the layer's
+    // large orbits (6x6 Fermi-Hubbard, threshold after every rotation: 8.3 ->
+    // 7.2 ms).  Otherwise tiles stay contiguous: an orbit's members, all kept,
+    // then share a block's cache lines (the kicked-Ising layers without threshold
+    // points lose 4 % when spread).
+    const uint32_t run = (!L.spread || tile == blockDim.x) ? tile
+                                                           : (tile < (uint32_t)kDLayerRun ? tile : (uint32_t)kDLayerRun);
+    const uint64_t step = (uint64_t)gridDim.x * tile;
+    if ((uint64_t)blockIdx.x * run >= n) return;  // block-uniform
     for (int t = threadIdx.x; t < 256; t += blockDim.x) spos[t] = L.pos[t];
     for (int t = threadIdx.x; t < L.nr; t += blockDim.x) {
         sg[t] = L.gates[t];
@@ -268,13 +284,15 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
             }
         }
     };
-    for (uint32_t b0 = blockIdx.x * tile; b0 < n; b0 += step) {
+    for (uint64_t b0 = 0; b0 + (uint64_t)blockIdx.x * run < n; b0 += step) {
         if (threadIdx.x == 0) s_cnt = 0;
         if (threadIdx.x <= kLayerMaxA) scount[threadIdx.x] = 0;
         __syncthreads();
         // ---- phase 1: own slot: copy if inactive, else claim the orbit ----
-        const uint32_t i = b0 + threadIdx.x;
-        if (threadIdx.x < tile && i < n) {
+        const uint64_t i64 =
+            b0 + ((uint64_t)(threadIdx.x / run) * gridDim.x + blockIdx.x) * run + threadIdx.x % run;
+        const uint32_t i = (uint32_t)i64;
+        if (threadIdx.x < tile && i64 < n) {
             uint64_t k[NP], pm[4];
             key_load<NP>(st.key, i, k);
             int a = 0;
