@@ -38,7 +38,7 @@ namespace ffx {
 // the kernel at 4 blocks of 256 threads per CU for keys of up to 4 words
 constexpr int kDLayerStage = 256;
 // consecutive slots per run of a block iteration's tile
-constexpr int kDLayerRun = 16;
+constexpr int kDLayerRun = 4;
 
 template <class Alg, class CP>
 struct DLayerGate {
@@ -248,8 +248,9 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
     // side by side and their orbits under the next layer are often all large; with
     // truncation inside the layer such an orbit leaves few members but costs its
     // full size, and contiguous tiles would hand a few blocks most of the layer's
-    // large orbits (6x6 Fermi-Hubbard, threshold after every rotation: 8.3 ->
-    // 7.2 ms).  Otherwise tiles stay contiguous: an orbit's members, all kept,
+    // large orbits (6x6 Fermi-Hubbard, threshold after every rotation: 8.3 ms with
+    // contiguous tiles, 7.2 ms with runs of 16 slots, 6.8 ms with runs of 4 or 1).
+    // Otherwise tiles stay contiguous: an orbit's members, all kept,
     // then share a block's cache lines (the kicked-Ising layers without threshold
     // points lose 4 % when spread).
     const uint32_t run = (!L.spread || tile == blockDim.x) ? tile
