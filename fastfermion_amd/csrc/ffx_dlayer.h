@@ -23,9 +23,9 @@
 // A(q) is again an orbit invariant (commutation is a bicharacter, so flipping q
 // by P_j leaves q's commutation with every P_k unchanged when P_j and P_k
 // commute), |A| <= 8 because the layer holds at most 8 rotations, and the 2^|A|
-// members are distinct because the axes are independent; with no private pivot
-// bits, the canonical member is the orbit's smallest key, found by a Gray-code
-// walk over the orbit.
+// members are distinct because the axes are independent.  With no private pivot
+// bits, the canonical member clears the active axes' coordinates of the key's
+// component in span{P}, read off with the axes' dual vectors (host-computed).
 //
 // Requires every live term's degree <= kLayerMaxA (host check as for layers).
 #pragma once
@@ -50,7 +50,8 @@ struct DLayerGate {
 template <int W, class CP, class Alg>
 struct DLayer {
     const DLayerGate<Alg, CP>* gates;  // nr records in layer order
-    const int16_t* pos;                // unit -> position in the layer, -1: none (256 entries)
+    const int16_t* pos;                // unit -> position in the layer, -1: none (256 entries);
+                                       // general layers: the axes' dual vectors (kLayerMaxA x NP words)
     int32_t* sd;                       // live-count change at each rotation (nr entries), kStripes copies
     uint32_t sd_stride;
     uint64_t* claim;                   // claim table (hmask + 1 entries)
@@ -147,6 +148,30 @@ __device__ __forceinline__ void dlayer_canon_min(const uint64_t (&k)[Alg::NP], u
     }
 }
 
+// Canonical member of a general layer's orbit from the axes' dual vectors
+// (parity(P_i & w_j) = [i == j]): k with the span{P} coordinate a_j =
+// parity(k & w_j) of every active axis j cleared, i.e. k ^ sum_{j active, a_j = 1} P_j.
+// Flipping k by an active axis toggles only that axis's coordinate, so every
+// member of the orbit gives the same key (A parities instead of a 2^A walk).
+template <class Alg, class CP>
+__device__ __forceinline__ void dlayer_canon_dual(const uint64_t (&k)[Alg::NP], uint64_t pm0,
+                                                  const uint64_t (*dual)[Alg::NP], const DLayerGate<Alg, CP>* sg,
+                                                  uint64_t (&ck)[Alg::NP]) {
+    constexpr int NP = Alg::NP;
+#pragma unroll
+    for (int q = 0; q < NP; ++q) ck[q] = k[q];
+    while (pm0) {
+        const int j = __builtin_ctzll(pm0);
+        pm0 &= pm0 - 1;
+        int par = 0;
+#pragma unroll
+        for (int q = 0; q < NP; ++q) par ^= __builtin_popcountll(k[q] & dual[j][q]);
+        if (par & 1)
+#pragma unroll
+            for (int q = 0; q < NP; ++q) ck[q] ^= sg[j].g.p[q];
+    }
+}
+
 // Canonical member: the pivot bit of every active axis cleared.
 template <class Alg, class CP>
 __device__ __forceinline__ void dlayer_canon(const uint64_t (&k)[Alg::NP], const uint64_t (&pm)[4],
@@ -218,7 +243,11 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
                                                          DLayer<Alg::W, CP, Alg> L, Epochs ep, uint32_t e) {
     constexpr int NP = Alg::NP;
     using val_t = typename CP::val_t;
-    __shared__ int16_t spos[256];
+    __shared__ union {
+        int16_t pos[256];                        // disjoint layers: unit -> position
+        uint64_t dual[kLayerMaxA][NP];           // general layers: dual vectors of the axes
+    } su;
+    int16_t* const spos = su.pos;
     __shared__ DLayerGate<Alg, CP> sg[kLayerMax];
     __shared__ uint64_t skey[kDLayerStage][NP];
     __shared__ val_t sval[kDLayerStage][K];
@@ -257,7 +286,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
                                                            : (tile < (uint32_t)kDLayerRun ? tile : (uint32_t)kDLayerRun);
     const uint64_t step = (uint64_t)gridDim.x * tile;
     if ((uint64_t)blockIdx.x * run >= n) return;  // block-uniform
-    for (int t = threadIdx.x; t < 256; t += blockDim.x) spos[t] = L.pos[t];
+    for (int t = threadIdx.x; t < 256; t += blockDim.x) spos[t] = L.pos[t];  // a plain copy of either form
     for (int t = threadIdx.x; t < L.nr; t += blockDim.x) {
         sg[t] = L.gates[t];
         ssd[t] = 0;
@@ -340,7 +369,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
             } else {
                 uint64_t ck[NP];
                 if (L.general)
-                    dlayer_canon_min<Alg, CP>(k, pack_positions(pm), a, sg, ck);
+                    dlayer_canon_dual<Alg, CP>(k, pm[0], su.dual, sg, ck);
                 else
                     dlayer_canon<Alg, CP>(k, pm, sg, ck);
                 const uint64_t h = hmix(lin_hash<NP>(ck));
@@ -371,7 +400,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
                         key_load<NP>(st.key, j, kj);
                         if (L.general) {
                             dlayer_active_all<Alg, CP>(kj, sg, L.nr, pmj, aj);
-                            dlayer_canon_min<Alg, CP>(kj, pack_positions(pmj), aj, sg, ckj);
+                            dlayer_canon_dual<Alg, CP>(kj, pmj[0], su.dual, sg, ckj);
                         } else {
                             dlayer_active<Alg, CP>(kj, L.umask, spos, sg, pmj, aj);
                             dlayer_canon<Alg, CP>(kj, pmj, sg, ckj);

@@ -1004,6 +1004,59 @@ struct Engine {
         return k >= (size_t)kLayerMin ? k : 0;
     }
 
+    // Dual vectors of a general layer's axes P_0..P_{nr-1} (linearly independent key
+    // vectors, nr <= kLayerMaxA): w_j with parity(P_i & w_j) = [i == j], supported on
+    // one pivot bit per axis.  For a key k, a_j = parity(k & w_j) are the coordinates of
+    // k's component in span{P}; clearing those of the active axes gives one member of
+    // k's orbit that every member agrees on.  False if the axes are dependent.
+    static bool general_duals(const AStep* a, int nr, uint64_t (*dual)[NP]) {
+        uint64_t v[kLayerMaxA][NP];
+        int piv[kLayerMaxA];
+        for (int b = 0; b < nr; ++b) {
+            for (int q = 0; q < NP; ++q) v[b][q] = a[b].g.p[q];
+            for (int c = 0; c < b; ++c)
+                if ((v[b][piv[c] >> 6] >> (piv[c] & 63)) & 1)
+                    for (int q = 0; q < NP; ++q) v[b][q] ^= v[c][q];
+            piv[b] = -1;
+            for (int q = 0; q < NP && piv[b] < 0; ++q)
+                if (v[b][q]) piv[b] = 64 * q + __builtin_ctzll(v[b][q]);
+            if (piv[b] < 0) return false;
+        }
+        // Q[i] (bit l) = bit piv[l] of P_i; Q is invertible over GF(2); R = Q^-1
+        uint32_t Q[kLayerMaxA], R[kLayerMaxA];
+        for (int i = 0; i < nr; ++i) {
+            Q[i] = 0;
+            R[i] = 1u << i;
+            for (int l = 0; l < nr; ++l) Q[i] |= (uint32_t)((a[i].g.p[piv[l] >> 6] >> (piv[l] & 63)) & 1) << l;
+        }
+        for (int c = 0; c < nr; ++c) {
+            int r = c;
+            while (r < nr && !((Q[r] >> c) & 1)) ++r;
+            if (r == nr) return false;
+            std::swap(Q[r], Q[c]);
+            std::swap(R[r], R[c]);
+            for (int i = 0; i < nr; ++i)
+                if (i != c && ((Q[i] >> c) & 1)) {
+                    Q[i] ^= Q[c];
+                    R[i] ^= R[c];
+                }
+        }
+        // Q R_rows = I as row operations: R (rows) = Q^-1, so (Q^-1)[l][j] = bit j of R[l];
+        // w_j = sum_l (Q^-1)[l][j] e_piv[l]  gives  parity(P_i & w_j) = (Q Q^-1)[i][j]
+        for (int j = 0; j < nr; ++j) {
+            for (int q = 0; q < NP; ++q) dual[j][q] = 0;
+            for (int l = 0; l < nr; ++l)
+                if ((R[l] >> j) & 1) dual[j][piv[l] >> 6] |= 1ull << (piv[l] & 63);
+        }
+        for (int i = 0; i < nr; ++i)
+            for (int j = 0; j < nr; ++j) {
+                int par = 0;
+                for (int q = 0; q < NP; ++q) par ^= __builtin_popcountll(a[i].g.p[q] & dual[j][q]) & 1;
+                if (par != (i == j ? 1 : 0)) return false;
+            }
+        return true;
+    }
+
     TopK topk_scratch() const {
         TopK tk;
         tk.hist = ctx->topk_buf;
@@ -1164,6 +1217,15 @@ struct Engine {
                         }
                         if (a.trunc_thresh) m3[3 * FFX_MAX_WORDS + (j >> 6)] |= 1ull << (j & 63);
                         if (a.trunc_xw) m3[3 * FFX_MAX_WORDS + 4 + (j >> 6)] |= 1ull << (j & 63);
+                    }
+                    if (layer_general[t]) {
+                        // a general layer finds its rotations without the position table: its
+                        // 512 bytes hold the axes' dual vectors instead
+                        static_assert(kLayerMaxA * NP * sizeof(uint64_t) <= 256 * sizeof(int16_t), "dual vectors");
+                        uint64_t dual[kLayerMaxA][NP];
+                        if (!general_duals(&steps[t], (int)layer_len[t], dual))
+                            return set_err(ctx, FFX_EHIP, "internal error: general layer with dependent axes");
+                        std::memcpy(pt, dual, layer_len[t] * NP * sizeof(uint64_t));
                     }
                     layer_masks.insert(layer_masks.end(), m3, m3 + kLM);
                     ++li;
