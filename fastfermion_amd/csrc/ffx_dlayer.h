@@ -270,18 +270,19 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
             tile = t;
             break;
         }
-    // layers with threshold points (L.spread), when every slot fits into one block
-    // iteration (tiles of at most 64 slots): the tile is taken in runs of kDLayerRun
-    // consecutive slots spread over the store (run j of block b at
-    // ((j * grid + b) * kDLayerRun)).  The fresh members of one orbit are appended
-    // side by side and their orbits under the next layer are often all large; with
-    // truncation inside the layer such an orbit leaves few members but costs its
-    // full size, and contiguous tiles would hand a few blocks most of the layer's
-    // large orbits (6x6 Fermi-Hubbard, threshold after every rotation: 8.3 ms with
-    // contiguous tiles, 7.2 ms with runs of 16 slots, 6.8 ms with runs of 4 or 1).
-    // Otherwise tiles stay contiguous: an orbit's members, all kept,
-    // then share a block's cache lines (the kicked-Ising layers without threshold
-    // points lose 4 % when spread).
+    // layers of at least 16 rotations with threshold points (L.spread), when every
+    // slot fits into one block iteration (tiles of at most 64 slots): the tile is
+    // taken in runs of kDLayerRun consecutive slots spread over the store (run j of
+    // block b at ((j * grid + b) * kDLayerRun)).  The fresh members of one orbit are
+    // appended side by side and their orbits under the next large layer are often
+    // all large; with truncation inside the layer such an orbit leaves few members
+    // but costs its full size, and contiguous tiles would hand a few blocks most of
+    // the layer's large orbits (6x6 Fermi-Hubbard, layers of 36-72 rotations,
+    // threshold after every rotation: 8.3 ms with contiguous tiles, 7.2 ms with runs
+    // of 16 slots, 6.8 ms with runs of 4 or 1).  Otherwise tiles stay contiguous: an
+    // orbit's members then tend to share a block's cache lines (the kicked-Ising
+    // layers of up to 10 rotations lose 4 % without threshold points and 11 % with
+    // one after every rotation when spread).
     const uint32_t run = (!L.spread || tile == blockDim.x) ? tile
                                                            : (tile < (uint32_t)kDLayerRun ? tile : (uint32_t)kDLayerRun);
     const uint64_t step = (uint64_t)gridDim.x * tile;

@@ -1352,7 +1352,7 @@ struct Engine {
                     L.max_xw = max_xw;
                     L.stamp = ctx->layer_stamp;
                     L.general = layer_general[t];
-                    L.spread = (L.tmask[0] | L.tmask[1] | L.tmask[2] | L.tmask[3]) != 0;
+                    L.spread = (L.tmask[0] | L.tmask[1] | L.tmask[2] | L.tmask[3]) != 0 && kw >= 16;
                     hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K>), dim3(win_grid), dim3(kBlock), 0, ctx->stream,
                                        store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L, ep, e);
                     CHK(launch_check());
