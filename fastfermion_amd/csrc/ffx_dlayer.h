@@ -297,6 +297,10 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
     int dd = 0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
+    // a layer without truncation points (a kernel argument, so the test is uniform)
+    // skips every per-member truncation check
+    const bool tr_any = ((L.tmask[0] | L.tmask[1] | L.tmask[2] | L.tmask[3]) |
+                         (L.xmask[0] | L.xmask[1] | L.xmask[2] | L.xmask[3])) != 0;
     // stage a fresh live member for appending (one global reservation per block iteration)
     auto stage_append = [&](const uint64_t (&key)[NP], const val_t (&v)[K]) {
         const uint32_t r = atomicAdd(&s_cnt, 1u);
@@ -458,6 +462,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
                 const bool alive0 = alive;
                 // truncation points in [lo, hi) on this lane's member
                 auto trunc1 = [&](int lo, int hi) {
+                    if (!tr_any) return;
                     if (!alive) return;
                     const int pt = first_in(L.tmask, lo, hi), px = first_in(L.xmask, lo, hi);
                     const bool bt = pt >= 0 && magnitude<CP, K>(v) <= L.mincoeff;
@@ -561,6 +566,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
             // truncation points in layer positions [lo, hi) applied to the orbit's current
             // values; erasures are counted at the first such point that erases
             auto truncate_range = [&](int lo, int hi) {
+                if (!tr_any) return;
                 const int pt = first_in(L.tmask, lo, hi), px = first_in(L.xmask, lo, hi);
                 if (pt < 0 && px < 0) return;
                 int kt = 0, kx = 0;
