@@ -399,17 +399,16 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
                         if ((e0 >> 40) != L.stamp) continue;  // overwritten meanwhile: look again
                     }
                     if (((e0 >> 32) & 0xFFull) == tag8) {
+                        // the claimant's key kj is in k's orbit iff reducing it by k's
+                        // active axes gives k's canonical member (the active set is an
+                        // orbit invariant, so kj's own is not needed)
                         const uint32_t j = (uint32_t)e0;
-                        uint64_t kj[NP], pmj[4], ckj[NP];
-                        int aj = 0;
+                        uint64_t kj[NP], ckj[NP];
                         key_load<NP>(st.key, j, kj);
-                        if (L.general) {
-                            dlayer_active_all<Alg, CP>(kj, sg, L.nr, pmj, aj);
-                            dlayer_canon_dual<Alg, CP>(kj, pmj[0], su.dual, sg, ckj);
-                        } else {
-                            dlayer_active<Alg, CP>(kj, L.umask, spos, sg, pmj, aj);
-                            dlayer_canon<Alg, CP>(kj, pmj, sg, ckj);
-                        }
+                        if (L.general)
+                            dlayer_canon_dual<Alg, CP>(kj, pm[0], su.dual, sg, ckj);
+                        else
+                            dlayer_canon<Alg, CP>(kj, pm, sg, ckj);
                         bool eq = true;
 #pragma unroll
                         for (int q = 0; q < NP; ++q) eq = eq && ckj[q] == ck[q];

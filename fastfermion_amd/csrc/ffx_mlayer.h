@@ -575,16 +575,16 @@ __global__ void __launch_bounds__(kBlock) k_mlayer_claim(Store<CP, K> st, const 
                             if ((e0 >> 40) != M.stamp) continue;  // overwritten meanwhile: look again
                         }
                         if (((e0 >> 32) & 0xFFull) == tag8) {
-                            // the claimant's orbit is this one iff its canonical key is ck; a
-                            // claimant of another instance never is (the tag is part of the key)
+                            // the claimant's key kj is in this orbit iff reducing it by this
+                            // slot's active axes gives ck (the active set is an orbit
+                            // invariant); a claimant of another instance never is (the tag is
+                            // part of the key)
                             const uint32_t j = (uint32_t)e0;
                             uint64_t kj[NP];
                             key_load<NP>(st.key, j, kj);
                             if (merged_instance<Alg>(kj, M.idshift, M.idmask) == inst) {
-                                int aj = 0;
                                 uint64_t ckj[NP];
-                                const uint64_t pmj = mgroup_active<Alg, CP>(kj, G, gt, gpos, aj);
-                                mgroup_canon<Alg, CP>(kj, pmj, aj, G.general != 0, gt, ckj);
+                                mgroup_canon<Alg, CP>(kj, pm, a, G.general != 0, gt, ckj);
                                 bool eq = true;
 #pragma unroll
                                 for (int q = 0; q < NP; ++q) eq = eq && ckj[q] == ck[q];
