@@ -64,7 +64,24 @@ struct DLayer {
     uint32_t stamp;  // this launch's stamp (24 bits, nonzero)
     int general;     // 1: at most kLayerMaxA commuting rotations with overlapping supports
     int spread;      // 1: small tiles taken in runs spread over the store (layers with threshold points)
+    int tflags;      // kTruncAny | kTruncXw | kTruncAll (dlayer_tflags)
 };
+
+constexpr int kTruncAny = 1;  // some threshold or X/Y-cap point in the layer
+constexpr int kTruncXw = 2;   // some X/Y-cap point
+constexpr int kTruncAll = 4;  // a threshold point after every rotation
+
+__host__ inline int dlayer_tflags(const uint64_t (&tmask)[4], const uint64_t (&xmask)[4], int nr) {
+    bool any = false, xw = false, all = true;
+    for (int u = 0; u < 4; ++u) {
+        any = any || tmask[u] || xmask[u];
+        xw = xw || xmask[u];
+        const int lo = 64 * u, hi = nr < 64 * (u + 1) ? nr : 64 * (u + 1);
+        const uint64_t want = hi <= lo ? 0ull : (hi - lo >= 64 ? ~0ull : ((1ull << (hi - lo)) - 1));
+        all = all && (tmask[u] & want) == want;
+    }
+    return (any ? kTruncAny : 0) | (xw ? kTruncXw : 0) | (all ? kTruncAll : 0);
+}
 
 // Units of a key (qubits in the Pauli support, Majorana indices), word w.
 template <class Alg>
@@ -237,7 +254,9 @@ __device__ __forceinline__ void dlayer_member(const uint64_t (&ck)[Alg::NP], uin
     }
 }
 
-template <class Alg, class CP, int K>
+// TR: the layer has truncation points (kTruncAny); without, every per-member
+// truncation check is compiled out.
+template <class Alg, class CP, int K, bool TR>
 __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, const typename CP::store_t* __restrict__ cin,
                                                          typename CP::store_t* __restrict__ cout,
                                                          DLayer<Alg::W, CP, Alg> L, Epochs ep, uint32_t e) {
@@ -297,10 +316,15 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
     int dd = 0;
     __syncthreads();
     const int wave = threadIdx.x >> 6, nwave = blockDim.x >> 6;
-    // a layer without truncation points (a kernel argument, so the test is uniform)
-    // skips every per-member truncation check
-    const bool tr_any = ((L.tmask[0] | L.tmask[1] | L.tmask[2] | L.tmask[3]) |
-                         (L.xmask[0] | L.xmask[1] | L.xmask[2] | L.xmask[3])) != 0;
+    // a layer without truncation points (kernel-argument flags, so the tests are
+    // uniform) skips every per-member truncation check; without X/Y caps, or with a
+    // threshold after every rotation, the search of the position masks is skipped too
+    constexpr bool tr_any = TR;
+    const bool xm_any = (L.tflags & kTruncXw) != 0;
+    const bool tm_all = (L.tflags & kTruncAll) != 0;
+    // first threshold / X/Y-cap position in [lo, hi), -1: none
+    auto first_t = [&](int lo, int hi) -> int { return tm_all ? (lo < hi ? lo : -1) : first_in(L.tmask, lo, hi); };
+    auto first_x = [&](int lo, int hi) -> int { return xm_any ? first_in(L.xmask, lo, hi) : -1; };
     // stage a fresh live member for appending (one global reservation per block iteration)
     auto stage_append = [&](const uint64_t (&key)[NP], const val_t (&v)[K]) {
         const uint32_t r = atomicAdd(&s_cnt, 1u);
@@ -343,8 +367,8 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
                 for (int kk = 0; kk < K; ++kk) c[kk] = CP::load(cin + (size_t)kk * CP::NS * cap, cap, i);
                 bool kill = false;
                 int at = -1;
-                if (!CP::dead(c[0])) {
-                    const int pt = first_in(L.tmask, 0, L.nr), px = first_in(L.xmask, 0, L.nr);
+                if (tr_any && !CP::dead(c[0])) {
+                    const int pt = first_t(0, L.nr), px = first_x(0, L.nr);
                     if (pt >= 0 && magnitude<CP, K>(c) <= L.mincoeff) {
                         kill = true;
                         at = pt;
@@ -463,7 +487,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
                 auto trunc1 = [&](int lo, int hi) {
                     if (!tr_any) return;
                     if (!alive) return;
-                    const int pt = first_in(L.tmask, lo, hi), px = first_in(L.xmask, lo, hi);
+                    const int pt = first_t(lo, hi), px = first_x(lo, hi);
                     const bool bt = pt >= 0 && magnitude<CP, K>(v) <= L.mincoeff;
                     const bool bx = px >= 0 && Alg::xweight(key) > L.max_xw;
                     if (bt && (!bx || pt <= px)) {
@@ -566,7 +590,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
             // values; erasures are counted at the first such point that erases
             auto truncate_range = [&](int lo, int hi) {
                 if (!tr_any) return;
-                const int pt = first_in(L.tmask, lo, hi), px = first_in(L.xmask, lo, hi);
+                const int pt = first_t(lo, hi), px = first_x(lo, hi);
                 if (pt < 0 && px < 0) return;
                 int kt = 0, kx = 0;
 #pragma unroll

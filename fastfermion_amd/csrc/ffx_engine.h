@@ -1353,8 +1353,15 @@ struct Engine {
                     L.stamp = ctx->layer_stamp;
                     L.general = layer_general[t];
                     L.spread = (L.tmask[0] | L.tmask[1] | L.tmask[2] | L.tmask[3]) != 0 && kw >= 16;
-                    hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K>), dim3(win_grid), dim3(kBlock), 0, ctx->stream,
-                                       store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L, ep, e);
+                    L.tflags = dlayer_tflags(L.tmask, L.xmask, L.nr);
+                    if (L.tflags & kTruncAny)
+                        hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K, true>), dim3(win_grid), dim3(kBlock), 0,
+                                           ctx->stream, store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L,
+                                           ep, e);
+                    else
+                        hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K, false>), dim3(win_grid), dim3(kBlock), 0,
+                                           ctx->stream, store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L,
+                                           ep, e);
                     CHK(launch_check());
                     if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev++ + 1], ctx->stream));
                     std::swap(ctx->coef, ctx->coef2);
