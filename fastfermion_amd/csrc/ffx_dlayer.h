@@ -130,11 +130,20 @@ __device__ __forceinline__ void dlayer_active(const uint64_t (&k)[Alg::NP], cons
 }
 
 // Positions of a general layer's rotations (nr <= kLayerMaxA) that anticommute with k.
+// A Pauli axis anticommutes only with strings it shares a qubit with, so a Pauli key
+// with no unit under the layer's axes (umask) has none.  (A Majorana axis of odd
+// degree can anticommute without overlap.)
 template <class Alg, class CP>
-__device__ __forceinline__ void dlayer_active_all(const uint64_t (&k)[Alg::NP], const DLayerGate<Alg, CP>* sg,
-                                                  int nr, uint64_t (&pm)[4], int& na) {
+__device__ __forceinline__ void dlayer_active_all(const uint64_t (&k)[Alg::NP], const uint64_t* umask,
+                                                  const DLayerGate<Alg, CP>* sg, int nr, uint64_t (&pm)[4], int& na) {
     pm[0] = pm[1] = pm[2] = pm[3] = 0;
     na = 0;
+    if constexpr (Alg::kPauli) {
+        uint64_t touch = 0;
+#pragma unroll
+        for (int w = 0; w < Alg::W; ++w) touch |= key_units<Alg>(k, w) & umask[w];
+        if (!touch) return;
+    }
     for (int j = 0; j < nr; ++j)
         if (Alg::anticommutes(k, sg[j].g)) {
             pm[0] |= 1ull << j;
@@ -356,7 +365,7 @@ __global__ void __launch_bounds__(kBlock, 4) k_dlayer_async(Store<CP, K> st, con
             key_load<NP>(st.key, i, k);
             int a = 0;
             if (L.general)
-                dlayer_active_all<Alg, CP>(k, sg, L.nr, pm, a);
+                dlayer_active_all<Alg, CP>(k, L.umask, sg, L.nr, pm, a);
             else
                 dlayer_active<Alg, CP>(k, L.umask, spos, sg, pm, a);
             if (a == 0) {
