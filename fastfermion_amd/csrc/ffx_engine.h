@@ -2269,6 +2269,7 @@ template <int W, class CP>
 struct MInstPlan {
     std::vector<MGroup<W>> groups;
     std::vector<DLayerGate<PauliAlg<W>, CP>> gates;
+    std::vector<uint64_t> duals;  // 2W words per gate record: the dual vector in general groups, else 0
     std::vector<uint8_t> pos;
     int64_t n_rot = 0;  // rotations of the schedule (records are shared, so gates.size() is fewer)
 };
@@ -2287,6 +2288,7 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
     using Gate = DLayerGate<PauliAlg<W>, CP>;
     P.groups.clear();
     P.gates.clear();
+    P.duals.clear();
     P.pos.clear();
     P.n_rot = (int64_t)sd.size();
     for (const auto& a : sd)
@@ -2371,6 +2373,12 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
             for (int w = 0; w < W; ++w)
                 for (uint64_t b = G.umask[w]; b; b &= b - 1) P.pos.push_back(upos[64 * w + __builtin_ctzll(b)]);
         {
+            uint64_t dual[kLayerMaxA][NP] = {};
+            if (general && !Eng::general_duals(&sd[t], (int)k, dual)) return false;
+            for (size_t j = 0; j < k; ++j)
+                for (int q = 0; q < NP; ++q) P.duals.push_back(general ? dual[j][q] : 0ull);
+        }
+        {
             // the position table is a function of the records and the kind, so equal
             // records of the same kind share it too
             const unsigned char* gb = (const unsigned char*)(P.gates.data() + G.gate_off);
@@ -2390,6 +2398,7 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
                 if (E.nr == G.nr && E.general == G.general &&
                     std::memcmp(P.gates.data() + E.gate_off, gb, nb) == 0) {
                     P.gates.resize(G.gate_off);
+                    P.duals.resize((size_t)G.gate_off * NP);
                     P.pos.resize(G.pos_off);
                     G.gate_off = E.gate_off;
                     G.pos_off = E.pos_off;
@@ -2450,7 +2459,8 @@ int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
     // device buffer: groups, gates, positions, gbase, glen, states | sums, counts, work stripes
     auto al = [](size_t v) { return (v + 255) & ~(size_t)255; };
     const size_t o_gate = al(std::max<size_t>(ng, 1) * sizeof(MGroup<W>));
-    const size_t o_pos = o_gate + al(std::max<size_t>(ngate, 1) * sizeof(Gate));
+    const size_t o_dual = o_gate + al(std::max<size_t>(ngate, 1) * sizeof(Gate));
+    const size_t o_pos = o_dual + al(std::max<size_t>(ngate, 1) * NP * sizeof(uint64_t));
     const size_t o_gbase = o_pos + al(std::max<size_t>(npos, 1));
     const size_t o_glen = o_gbase + al(ne * 4);
     const size_t o_state = o_glen + al(ne * 4);
@@ -2488,6 +2498,8 @@ int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
                 }
                 std::memcpy(H + o_gate + (size_t)gate_base[b] * sizeof(Gate), P.gates.data(),
                             P.gates.size() * sizeof(Gate));
+                std::memcpy(H + o_dual + (size_t)gate_base[b] * NP * sizeof(uint64_t), P.duals.data(),
+                            P.duals.size() * sizeof(uint64_t));
                 if (!P.pos.empty()) std::memcpy(H + o_pos + pos_base[b], P.pos.data(), P.pos.size());
                 const ffx_instance& in = inst[elig[b]];
                 uint64_t* st = (uint64_t*)(H + o_state) + b * W;
@@ -2525,6 +2537,7 @@ int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
     MLayer<Alg, CP> M{};
     M.groups = (const MGroup<W>*)mb;
     M.gates = (const Gate*)(mb + o_gate);
+    M.duals = (const uint64_t*)(mb + o_dual);
     M.pos = (const uint8_t*)(mb + o_pos);
     M.gbase = (const uint32_t*)(mb + o_gbase);
     M.glen = (const uint32_t*)(mb + o_glen);

@@ -51,6 +51,7 @@ struct MLayer {
     const uint32_t* gbase;            // first group of instance i
     const uint32_t* glen;             // groups of instance i
     const DLayerGate<Alg, CP>* gates;  // gate records, group by group
+    const uint64_t* duals;            // general groups: NP-word dual vector of each gate record
     const uint8_t* pos;               // unit -> position bytes
     uint64_t* claim;                  // claim table (hmask + 1 entries, stamped)
     unsigned long long* work;         // per stripe (kStripes copies, 16 words apart): updates, outs
@@ -136,14 +137,17 @@ __device__ __forceinline__ uint64_t mgroup_active(const uint64_t (&k)[Alg::NP], 
     return pm;
 }
 
-// Canonical member of k's orbit: the smallest key (general groups), or the key
-// with the private pivot bit of every active axis cleared (disjoint groups).
+// Canonical member of k's orbit: the key with the active axes' span coordinates
+// cleared, read off with the group's dual vectors (general groups, as for one
+// circuit's general layers), or with the private pivot bit of every active axis
+// cleared (disjoint groups).
 template <class Alg, class CP>
-__device__ __forceinline__ void mgroup_canon(const uint64_t (&k)[Alg::NP], uint64_t pm, int a, bool general,
-                                             const DLayerGate<Alg, CP>* gt, uint64_t (&ck)[Alg::NP]) {
+__device__ __forceinline__ void mgroup_canon(const uint64_t (&k)[Alg::NP], uint64_t pm, bool general,
+                                             const DLayerGate<Alg, CP>* gt, const uint64_t* duals,
+                                             uint64_t (&ck)[Alg::NP]) {
     constexpr int NP = Alg::NP;
     if (general) {
-        dlayer_canon_min<Alg, CP>(k, pack64(pm), a, gt, ck);
+        dlayer_canon_dual<Alg, CP>(k, pm, (const uint64_t(*)[NP])duals, gt, ck);
         return;
     }
 #pragma unroll
@@ -280,7 +284,7 @@ __device__ __forceinline__ void mlayer_rebuild(const uint64_t* keys, const MLaye
     G = M.groups[M.gbase[inst] + M.step];
     gt = M.gates + G.gate_off;
     const uint64_t pm = mgroup_active<Alg, CP>(kl, G, gt, M.pos + G.pos_off, a);
-    mgroup_canon<Alg, CP>(kl, pm, a, G.general != 0, gt, ck);
+    mgroup_canon<Alg, CP>(kl, pm, G.general != 0, gt, M.duals + (size_t)G.gate_off * Alg::NP, ck);
     ap = pack64(pm);
 }
 
@@ -552,7 +556,7 @@ __global__ void __launch_bounds__(kBlock) k_mlayer_claim(Store<CP, K> st, const 
                                   CP::load(cin + (size_t)kk * CP::NS * cap, cap, i));
                 } else {
                     uint64_t ck[NP];
-                    mgroup_canon<Alg, CP>(k, pm, a, G.general != 0, gt, ck);
+                    mgroup_canon<Alg, CP>(k, pm, G.general != 0, gt, M.duals + (size_t)G.gate_off * NP, ck);
                     const uint64_t h = hmix(lin_hash<NP>(ck));
                     const uint64_t tag8 = (h >> 56) & 0xFFull;
                     const uint64_t mine = ((uint64_t)M.stamp << 40) | (tag8 << 32) | (uint64_t)i;
@@ -584,7 +588,7 @@ __global__ void __launch_bounds__(kBlock) k_mlayer_claim(Store<CP, K> st, const 
                             key_load<NP>(st.key, j, kj);
                             if (merged_instance<Alg>(kj, M.idshift, M.idmask) == inst) {
                                 uint64_t ckj[NP];
-                                mgroup_canon<Alg, CP>(kj, pm, a, G.general != 0, gt, ckj);
+                                mgroup_canon<Alg, CP>(kj, pm, G.general != 0, gt, M.duals + (size_t)G.gate_off * NP, ckj);
                                 bool eq = true;
 #pragma unroll
                                 for (int q = 0; q < NP; ++q) eq = eq && ckj[q] == ck[q];
