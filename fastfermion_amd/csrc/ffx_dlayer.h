@@ -151,29 +151,6 @@ __device__ __forceinline__ void dlayer_active_all(const uint64_t (&k)[Alg::NP], 
         }
 }
 
-// Canonical member of a general layer's orbit: the smallest key (last word most
-// significant) of k ^ span{active axes}, visited in Gray-code order.
-template <class Alg, class CP>
-__device__ __forceinline__ void dlayer_canon_min(const uint64_t (&k)[Alg::NP], uint64_t ap, int a,
-                                                 const DLayerGate<Alg, CP>* sg, uint64_t (&ck)[Alg::NP]) {
-    constexpr int NP = Alg::NP;
-    uint64_t cur[NP];
-#pragma unroll
-    for (int q = 0; q < NP; ++q) ck[q] = cur[q] = k[q];
-    for (int i = 1; i < (1 << a); ++i) {
-        const int pq = (int)((ap >> (8 * __builtin_ctz(i))) & 0xFFu);
-#pragma unroll
-        for (int q = 0; q < NP; ++q) cur[q] ^= sg[pq].g.p[q];
-        int c = 0;  // sign of cur - ck, decided by the most significant differing word
-#pragma unroll
-        for (int q = 0; q < NP; ++q)
-            if (cur[q] != ck[q]) c = cur[q] < ck[q] ? -1 : 1;
-        if (c < 0)
-#pragma unroll
-            for (int q = 0; q < NP; ++q) ck[q] = cur[q];
-    }
-}
-
 // Canonical member of a general layer's orbit from the axes' dual vectors
 // (parity(P_i & w_j) = [i == j]): k with the span{P} coordinate a_j =
 // parity(k & w_j) of every active axis j cleared, i.e. k ^ sum_{j active, a_j = 1} P_j.
