@@ -1127,7 +1127,10 @@ struct Engine {
         int rot_grid = 2 * ctx->n_cu;
         if (const char* ev = getenv("FFX_ROT_GRID"))
             if (atoi(ev) > 0) rot_grid = atoi(ev);
-        int win_grid = 6 * ctx->n_cu;
+        // one resident round of the layer and window kernels (4 blocks of 256 threads per
+        // CU): a round and a half left a tail of late blocks (kicked Ising per-gate
+        // 8.9 -> 8.5 ms, Fermi-Hubbard 6.1 -> 6.0 ms; the batch launches keep 6 per CU)
+        int win_grid = 4 * ctx->n_cu;
         if (const char* ev = getenv("FFX_WIN_GRID"))
             if (atoi(ev) > 0) win_grid = atoi(ev);
         int max_window = kWin;
