@@ -2565,6 +2565,10 @@ int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
     unsigned grid = 6 * (unsigned)ctx->n_cu;
     if (const char* ev = getenv("FFX_WIN_GRID"))
         if (atoi(ev) > 0) grid = (unsigned)atoi(ev);
+    // the replay launch: one resident round at its occupancy (3 blocks per CU for keys of
+    // up to 3 words, else 2); the listed passes are spread over whatever grid runs them
+    // (C4: 77 -> 75 ms against the claim launch's 6 per CU)
+    const unsigned rgrid = (Alg::NP <= 6 ? 3u : 2u) * (unsigned)ctx->n_cu;
     // the store: as large as the last call's (a batch of the same shape needs the same),
     // doubled from the start whenever an append does not fit
     size_t want = std::max<size_t>(std::max<size_t>(4 * nt + 4096, 1 << 16), ctx->mlayer_cap_hint);
@@ -2628,7 +2632,7 @@ int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
             hipLaunchKernelGGL((k_mlayer_claim<Alg, CP, K>), dim3(grid), dim3(kBlock), 0, ctx->stream, eng.store(),
                                (const store_t*)ctx->coef, (store_t*)ctx->coef2, M, ep, e);
             CHK(eng.launch_check());
-            hipLaunchKernelGGL((k_mlayer_replay<Alg, CP, K>), dim3(grid), dim3(kBlock), 0, ctx->stream, eng.store(),
+            hipLaunchKernelGGL((k_mlayer_replay<Alg, CP, K>), dim3(rgrid), dim3(kBlock), 0, ctx->stream, eng.store(),
                                (const store_t*)ctx->coef, (store_t*)ctx->coef2, M, ep, e);
             CHK(eng.launch_check());
             std::swap(ctx->coef, ctx->coef2);
