@@ -95,7 +95,7 @@ class BatchTuning(ctypes.Structure):
     _fields_ = [("block", ctypes.c_int32), ("window", ctypes.c_int32), ("slots", ctypes.c_int64),
                 ("ms", ctypes.c_double), ("candidates", ctypes.c_int32), ("sample", ctypes.c_int32),
                 ("merged", ctypes.c_int32), ("compact_every", ctypes.c_int32), ("group_max", ctypes.c_int32),
-                ("reserved", ctypes.c_int32)]
+                ("split", ctypes.c_int32)]
 
 
 class Profile(ctypes.Structure):

@@ -57,6 +57,7 @@ int ffx_ctx_destroy(ffx_ctx* ctx) {
     if (!ctx) return FFX_OK;
     (void)hipSetDevice(ctx->device);
     if (ctx->stream) (void)hipStreamSynchronize(ctx->stream);
+    if (ctx->aux) ffx_ctx_destroy(ctx->aux);
     free_store(ctx);
     dev_free(ctx->ctrl);
     dev_free(ctx->counts);
@@ -393,15 +394,17 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     const int blocks[2] = {256, 512};
     const int windows[2] = {3, 4};
     const int64_t budgets[2] = {8192, 32768};
-    // candidates: the layer-group layout per compaction interval and general group size,
-    // the window-step layout, then the one-workgroup layout
+    // candidates: the layer-group layout per compaction interval, general group size and
+    // split (long schedules as a second store beside the rest, or all in one), the
+    // window-step layout, then the one-workgroup layout
     std::vector<BatchConf> cands;
-    for (int ce : {2, 4, 8})
-        for (int gm : {4, 6, 8}) cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 1, ce, gm});
-    cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 2, 0, 0});
+    for (int ce : {2, 4})
+        for (int gm : {6, 8})
+            for (int sp : {0, kMSplitDefault}) cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 1, ce, gm, sp});
+    cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 2, 0, 0, 0});
     for (int bl : blocks)
         for (int wi : windows)
-            for (int64_t sl : budgets) cands.push_back(BatchConf{bl, wi, sl, 0, 0, 0});
+            for (int64_t sl : budgets) cands.push_back(BatchConf{bl, wi, sl, 0, 0, 0, 0});
     double best_ms = 1e300;
     BatchConf best{};
     int32_t tried = 0;
@@ -431,6 +434,7 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     t.merged = best.merged;
     t.compact_every = best.compact_every;
     t.group_max = best.group_max;
+    t.split = best.split;
     bool replaced = false;
     for (auto& e : ctx->batch_tuned)
         if (e.W == t.W && e.prec == t.prec && e.K == t.K) {
@@ -444,7 +448,7 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     out->merged = best.merged;
     out->compact_every = best.compact_every;
     out->group_max = best.group_max;
-    out->reserved = 0;
+    out->split = best.split;
     out->ms = best_ms;
     out->candidates = tried;
     out->sample = m;
