@@ -2041,8 +2041,9 @@ inline int mono_degree_mod4_r(const uint64_t* m, int W) {
 // are not eligible or outgrow their slot budget run on the device-wide engine.
 // Configuration of a batched launch: threads per instance workgroup, maximum
 // window length, slot budget per instance (0 in a field: the tuned or default value).
-// default split of the layer-group layout (percent of the longest schedule)
-constexpr int kMSplitDefault = 60;
+// term stores the layer-group layout cuts a batch into by schedule length (1..4), each
+// on its own stream, by default
+constexpr int kMSplitDefault = 4;
 
 struct BatchConf {
     int block = 0;
@@ -2052,8 +2053,8 @@ struct BatchConf {
                              // window steps (ffx_merged.h); 0: one workgroup each (ffx_batch.h)
     int compact_every = 0;   // layer-group steps between compaction checks
     int group_max = 0;       // most commuting rotations with overlapping supports in a group
-    int split = -1;          // layer-group layout: instances whose schedules run past split % of the
-                             // longest form a second term store on a second stream (0: off; -1: default)
+    int split = -1;          // layer-group layout: term stores (1..4) the batch is cut into by schedule
+                             // length, each on its own stream (-1: default)
 };
 
 // Merged batch (ffx_merged.h): the instances elig[0..ne) in one term store, one
@@ -2747,7 +2748,8 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
         if (conf->split >= 0) cf.split = conf->split;
     }
     if (cf.split < 0) cf.split = kMSplitDefault;
-    if (const char* ev = getenv("FFX_MSPLIT")) cf.split = std::max(0, std::min(100, atoi(ev)));
+    if (const char* ev = getenv("FFX_MSPLIT")) cf.split = atoi(ev);
+    cf.split = std::max(1, std::min(4, cf.split));
     if (const char* ev = getenv("FFX_BATCH_MERGED")) cf.merged = std::max(0, std::min(2, atoi(ev)));
     if (slots > 0) cf.slots = slots;
     if (const char* ev = getenv("FFX_BATCH_BLOCK")) {
@@ -2840,56 +2842,72 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     bool all_grouped = try_ml && !elig.empty();
     for (int32_t i : elig) all_grouped = all_grouped && !have_windows[i];
     if (all_grouped) {
-        // the instances whose schedules run past split % of the longest go to a second
-        // merged store on a second stream, beside the others: the short part's heavy
-        // steps then overlap the long part's latency-bound tail (C4: 75 -> 68 ms at 60 %)
-        std::vector<int32_t> part[2];
+        // cut by schedule length into cf.split term stores, each on its own stream: the
+        // short instances' heavy steps then overlap the long ones' latency-bound tails
+        // (C4: one store 75 ms; cut at 60 % of the longest schedule 68 ms; at 45/70 %
+        // 69 ms; at 30/55/80 % 65 ms)
+        static const std::vector<int> kCuts[4] = {{}, {60}, {45, 70}, {30, 55, 80}};
+        const std::vector<int>& cuts = kCuts[cf.split - 1];
+        const int np = (int)cuts.size() + 1;
+        std::vector<std::vector<int32_t>> part((size_t)np);
         size_t lmax = 0;
         for (int32_t i : elig) lmax = std::max(lmax, plans[i].groups.size());
-        for (int32_t i : elig)
-            part[cf.split > 0 && 100 * plans[i].groups.size() > (size_t)cf.split * lmax ? 1 : 0].push_back(i);
-        if (part[0].empty() || part[1].empty()) {
-            part[0] = elig;
-            part[1].clear();
+        for (int32_t i : elig) {
+            int h = 0;
+            while (h < np - 1 && 100 * plans[i].groups.size() > (size_t)cuts[(size_t)h] * lmax) ++h;
+            part[(size_t)h].push_back(i);
         }
-        uint64_t used[2][W] = {};
-        for (int h = 0; h < 2; ++h)
+        part.erase(std::remove_if(part.begin(), part.end(), [](const std::vector<int32_t>& v) { return v.empty(); }),
+                   part.end());
+        std::vector<std::array<uint64_t, W>> used(part.size());
+        for (size_t h = 0; h < part.size(); ++h) {
+            used[h].fill(0);
             for (int32_t i : part[h])
                 for (int w = 0; w < W; ++w) used[h][w] |= used_of[i][w];
-        if (part[1].empty()) {
-            CHK((run_pauli_mlayer<W, CP, K>(ctx, inst, part[0], plans, used[0], o, cf.compact_every, out_expect,
-                                            out_terms, merged_ran)));
-        } else {
-            if (!ctx->aux) {
-                const int rc = ffx_ctx_create(ctx->device, &ctx->aux);
+        }
+        // contexts: this one for the first part, then its chain of second contexts
+        std::vector<ffx_ctx*> cx(part.size(), ctx);
+        for (size_t h = 1; h < part.size(); ++h) {
+            if (!cx[h - 1]->aux) {
+                const int rc = ffx_ctx_create(ctx->device, &cx[h - 1]->aux);
                 if (rc != FFX_OK) return set_err(ctx, rc, "batch: second context for the long schedules");
             }
-            ffx_ctx* ax = ctx->aux;
-            ax->prof = ffx_profile{};
-            bool ran1 = false;
-            int rc1 = FFX_OK;
-            std::thread side([&]() {
+            cx[h] = cx[h - 1]->aux;
+            cx[h]->prof = ffx_profile{};
+        }
+        std::vector<int> rcs(part.size(), FFX_OK);
+        std::vector<char> ran(part.size(), 0);
+        auto run_part = [&](size_t h) {
+            uint64_t u[W];
+            for (int w = 0; w < W; ++w) u[w] = used[h][w];
+            bool r = false;
+            rcs[h] = run_pauli_mlayer<W, CP, K>(cx[h], inst, part[h], plans, u, o, cf.compact_every, out_expect,
+                                                out_terms, r);
+            ran[h] = r ? 1 : 0;
+        };
+        std::vector<std::thread> side;
+        for (size_t h = 1; h < part.size(); ++h)
+            side.emplace_back([&, h]() {
                 if (hipSetDevice(ctx->device) != hipSuccess) {
-                    rc1 = FFX_EHIP;
+                    rcs[h] = FFX_EHIP;
                     return;
                 }
-                rc1 = run_pauli_mlayer<W, CP, K>(ax, inst, part[1], plans, used[1], o, cf.compact_every, out_expect,
-                                                 out_terms, ran1);
+                run_part(h);
             });
-            bool ran0 = false;
-            const int rc0 = run_pauli_mlayer<W, CP, K>(ctx, inst, part[0], plans, used[0], o, cf.compact_every,
-                                                       out_expect, out_terms, ran0);
-            side.join();
-            if (rc1 != FFX_OK) return set_err(ctx, rc1, "batch (long schedules): " + ax->err);
-            CHK(rc0);
-            merged_ran = ran0 && ran1;
-            ctx->prof.updates += ax->prof.updates;
-            ctx->prof.outs += ax->prof.outs;
-            ctx->prof.n_gates += ax->prof.n_gates;
-            ctx->prof.peak_terms += ax->prof.peak_terms;
-            ctx->prof.run_launches = std::max(ctx->prof.run_launches, ax->prof.run_launches);
-            ctx->prof.compactions += ax->prof.compactions;
-            ctx->prof.launches += ax->prof.launches;
+        run_part(0);
+        for (auto& th : side) th.join();
+        CHK(rcs[0]);
+        merged_ran = ran[0] != 0;
+        for (size_t h = 1; h < part.size(); ++h) {
+            if (rcs[h] != FFX_OK) return set_err(ctx, rcs[h], "batch (long schedules): " + cx[h]->err);
+            merged_ran = merged_ran && ran[h];
+            ctx->prof.updates += cx[h]->prof.updates;
+            ctx->prof.outs += cx[h]->prof.outs;
+            ctx->prof.n_gates += cx[h]->prof.n_gates;
+            ctx->prof.peak_terms += cx[h]->prof.peak_terms;
+            ctx->prof.run_launches = std::max(ctx->prof.run_launches, cx[h]->prof.run_launches);
+            ctx->prof.compactions += cx[h]->prof.compactions;
+            ctx->prof.launches += cx[h]->prof.launches;
         }
     }
     if (!merged_ran) {

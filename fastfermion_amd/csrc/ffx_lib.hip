@@ -395,16 +395,16 @@ int ffx_batch_autotune(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, i
     const int windows[2] = {3, 4};
     const int64_t budgets[2] = {8192, 32768};
     // candidates: the layer-group layout per compaction interval, general group size and
-    // split (long schedules as a second store beside the rest, or all in one), the
-    // window-step layout, then the one-workgroup layout
+    // number of term stores cut by schedule length (1, 2 or 4, each on its own stream),
+    // the window-step layout, then the one-workgroup layout
     std::vector<BatchConf> cands;
     for (int ce : {2, 4})
         for (int gm : {6, 8})
-            for (int sp : {0, kMSplitDefault}) cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 1, ce, gm, sp});
-    cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 2, 0, 0, 0});
+            for (int sp : {1, 2, 4}) cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 1, ce, gm, sp});
+    cands.push_back(BatchConf{kBlockBatch, kWin, 32768, 2, 0, 0, 1});
     for (int bl : blocks)
         for (int wi : windows)
-            for (int64_t sl : budgets) cands.push_back(BatchConf{bl, wi, sl, 0, 0, 0, 0});
+            for (int64_t sl : budgets) cands.push_back(BatchConf{bl, wi, sl, 0, 0, 0, 1});
     double best_ms = 1e300;
     BatchConf best{};
     int32_t tried = 0;

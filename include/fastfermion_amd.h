@@ -206,8 +206,8 @@ typedef struct ffx_batch_tuning {
                                instance (block and slots apply to 0 only) */
     int32_t compact_every;  /* layout 1: group steps between device-decided compaction checks */
     int32_t group_max;      /* layout 1: most commuting rotations with overlapping supports in a group (1..8) */
-    int32_t split;          /* layout 1: instances whose schedules run past this percent of the longest
-                               form a second term store on a second stream (0: all in one store) */
+    int32_t split;          /* layout 1: term stores (1..4) the batch is cut into by schedule length, each
+                               on its own stream, so that long schedules run beside short ones */
 } ffx_batch_tuning;
 
 /* Autotuner of batched propagation (no reference counterpart): times the

@@ -159,6 +159,35 @@ def test_batch_groups_structured_circuits(ctx, monkeypatch, W, nq_hi, kw):
     assert ctx.profile()["updates"] == upd
 
 
+@pytest.mark.parametrize("split", ["1", "2", "4"])
+def test_batch_groups_cut_by_schedule_length(ctx, monkeypatch, split):
+    """The layer-group layout cut into 1, 2 or 4 term stores by schedule length, each
+    on its own stream and run side by side: every instance's <O> and live term count
+    equal the oracle's whatever the cut, and the update counter equals the oracle's
+    sum (the stores' counters are added)."""
+    monkeypatch.setenv("FFX_BATCH_MERGED", "1")
+    monkeypatch.setenv("FFX_MSPLIT", split)
+    rng = np.random.default_rng(4242)
+    W = 2
+    items, refs, upd = [], [], 0
+    for i in range(24):
+        nq = int(rng.integers(20, 100))
+        g = structured_circuit(rng, nq, int(rng.integers(2, 16)))  # schedules of very different lengths
+        q = int(rng.integers(0, nq))
+        x, z, c = wl.single_pauli("Z", [q], nq)
+        x, z = pad(x, W), pad(z, W)
+        items.append((g, x, z, c, None))
+        r = pyffo.pauli_propagate(g, W, x, z, c, maxdegree=8, mincoeff=1e-4, batched=False)
+        refs.append((pyffo.pauli_expect(r, [0, 0]), len(r.c)))
+        upd += r.updates
+    ex, nt, fb = ctx.pauli_propagate_batch(items, W, maxdegree=8, mincoeff=1e-4, batched=False)
+    assert fb == 0
+    for i, (e, n) in enumerate(refs):
+        assert abs(ex[i] - e) < 1e-12, (split, i)
+        assert nt[i] == n, (split, i)
+    assert ctx.profile()["updates"] == upd
+
+
 def test_batch_instance_numbers_that_do_not_fit(ctx, monkeypatch):
     """64 one-word instances on 60 qubits: 6 bits of instance number do not fit above
     qubit 59, so the merged layouts step aside for the one-workgroup layout, with the
@@ -295,7 +324,7 @@ def test_autotune_keeps_results(ctx):
     assert t["block"] in (256, 512) and 1 <= t["window"] <= 4 and t["slots"] >= 1024
     assert t["merged"] in (0, 1, 2)
     if t["merged"] == 1:
-        assert t["compact_every"] in (2, 4) and t["group_max"] in (6, 8) and t["split"] in (0, 60)
-    assert t["candidates"] == 17 and t["sample"] == 16 and t["ms"] > 0
+        assert t["compact_every"] in (2, 4) and t["group_max"] in (6, 8) and t["split"] in (1, 2, 4)
+    assert t["candidates"] == 21 and t["sample"] == 16 and t["ms"] > 0
     got = ctx.pauli_propagate_batch(items, W, **kw)
     assert np.max(np.abs(got[0] - want[0])) < 1e-13 and np.array_equal(got[1], want[1])
