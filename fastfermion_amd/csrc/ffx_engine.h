@@ -2328,9 +2328,10 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
                 ++kd;
             }
         }
-        // run of pairwise commuting rotations with linearly independent axes
+        // run of pairwise commuting rotations with linearly independent axes (not needed
+        // when the disjoint run already reaches gmax: the disjoint one wins ties)
         size_t kg = 0;
-        {
+        if (!(kd >= 2 && kd >= gmax)) {
             uint64_t basis[kLayerMaxA][NP];
             int pivot[kLayerMaxA];
             int nb = 0;
