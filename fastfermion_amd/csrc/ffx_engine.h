@@ -1131,12 +1131,14 @@ struct Engine {
         int rot_grid = 2 * ctx->n_cu;
         if (const char* ev = getenv("FFX_ROT_GRID"))
             if (atoi(ev) > 0) rot_grid = atoi(ev);
-        // one resident round of the layer and window kernels (4 blocks of 256 threads per
-        // CU): a round and a half left a tail of late blocks (kicked Ising per-gate
-        // 8.9 -> 8.5 ms, Fermi-Hubbard 6.1 -> 6.0 ms; the batch launches keep 6 per CU)
-        int win_grid = 4 * ctx->n_cu;
+        // the layer kernels launch one resident round (4 blocks of 256 threads per CU): a
+        // round and a half left a tail of late blocks (kicked Ising per-gate 8.9 -> 8.5 ms,
+        // Fermi-Hubbard 6.1 -> 6.0 ms).  The window kernel keeps 6 per CU: on 3.7e7 live
+        // terms (maxdegree 16, no layers) 4 per CU took 1.63 s against 1.32 s.
+        int win_grid = 6 * ctx->n_cu;
+        int layer_grid = 4 * ctx->n_cu;
         if (const char* ev = getenv("FFX_WIN_GRID"))
-            if (atoi(ev) > 0) win_grid = atoi(ev);
+            if (atoi(ev) > 0) win_grid = layer_grid = atoi(ev);
         int max_window = kWin;
         if (const char* ev = getenv("FFX_WINDOW")) max_window = std::min(kWin, std::max(1, atoi(ev)));
         // layers of single-qubit rotations (one launch each), planned greedily from the
@@ -1362,11 +1364,11 @@ struct Engine {
                     L.spread = (L.tmask[0] | L.tmask[1] | L.tmask[2] | L.tmask[3]) != 0 && kw >= 16;
                     L.tflags = dlayer_tflags(L.tmask, L.xmask, L.nr);
                     if (L.tflags & kTruncAny)
-                        hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K, true>), dim3(win_grid), dim3(kBlock), 0,
+                        hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K, true>), dim3(layer_grid), dim3(kBlock), 0,
                                            ctx->stream, store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L,
                                            ep, e);
                     else
-                        hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K, false>), dim3(win_grid), dim3(kBlock), 0,
+                        hipLaunchKernelGGL((k_dlayer_async<Alg, CP, K, false>), dim3(layer_grid), dim3(kBlock), 0,
                                            ctx->stream, store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L,
                                            ep, e);
                     CHK(launch_check());
@@ -1405,7 +1407,7 @@ struct Engine {
                         L.maxdeg = maxdeg;
                         L.max_xw = max_xw;
                         L.stamp = ctx->layer_stamp;
-                        hipLaunchKernelGGL((k_layer_async<Alg, CP, K>), dim3(win_grid), dim3(kBlock), 0, ctx->stream,
+                        hipLaunchKernelGGL((k_layer_async<Alg, CP, K>), dim3(layer_grid), dim3(kBlock), 0, ctx->stream,
                                            store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L, ep, e);
                         CHK(launch_check());
                         if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev++ + 1], ctx->stream));
