@@ -2580,7 +2580,7 @@ int run_pauli_mlayer(ffx_ctx* ctx, const ffx_instance* inst, const std::vector<i
         if (atoi(ev) > 0) grid = (unsigned)atoi(ev);
     // the replay launch: one resident round at its occupancy (3 blocks per CU for keys of
     // up to 3 words, else 2); the listed passes are spread over whatever grid runs them
-    // (C4: 77 -> 75 ms against the claim launch's 6 per CU)
+    // (512-instance kicked-Ising batch: 77 -> 75 ms against the claim launch's 6 per CU)
     const unsigned rgrid = (Alg::NP <= 6 ? 3u : 2u) * (unsigned)ctx->n_cu;
     // the store: as large as the last call's (a batch of the same shape needs the same),
     // doubled from the start whenever an append does not fit
@@ -2847,7 +2847,8 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     if (all_grouped) {
         // cut by schedule length into cf.split term stores, each on its own stream: the
         // short instances' heavy steps then overlap the long ones' latency-bound tails
-        // (C4: one store 75 ms; cut at 60 % of the longest schedule 68 ms; at 45/70 %
+        // (512-instance kicked-Ising batch: one store 75 ms; cut at 60 % of the longest
+        // schedule 68 ms; at 45/70 %
         // 69 ms; at 30/55/80 % 65 ms)
         static const std::vector<int> kCuts[4] = {{}, {60}, {45, 70}, {30, 55, 80}};
         const std::vector<int>& cuts = kCuts[cf.split - 1];

@@ -1,19 +1,18 @@
-"""Parity at the benchmark configurations' full sizes (device vs the CPU oracle).
+"""Parity at the workloads' full sizes (device vs the CPU oracle).
 
-  C2  10x10 kicked Ising, 20 layers, Z_55, maxdegree 8, mincoeff 1e-5, FP64, both
-      schedules: keys identical and coefficients bit-identical to the oracle's
-      key-sharded run (src/pauli/propagate.h:157-264, 297-493).
-  C3  the same circuit in FP32 and FP32 compensated: |d<O>|, relative coefficient
-      error on common keys and the key-set symmetric difference against FP64.
-  C4  512 uneven kicked-Ising instances (L 6..12, so up to 144 qubits and 3 key
-      words; depth 4..24) batched in every layout (one term store for all in
-      layer-group or window steps, one workgroup each), FP64 and FP32 compensated:
-      every instance's <O> and live term count against the oracle run per
-      instance.
-  C5  the 6x6 Fermi-Hubbard Majorana circuit (20 Trotter steps, 6960 rotations) in
-      both term orders, bit-identical to the serial Majorana oracle (src/majorana_propagation.h:18-33);
-      and the learned pruner on the 30-layer kicked-Ising and the Fermi-Hubbard
-      circuits within |d<O>| <= 1e-3 of the oracle.
+* 10x10 kicked Ising, 20 layers, Z_55, maxdegree 8, mincoeff 1e-5, FP64, both
+  schedules: keys identical and coefficients bit-identical to the oracle's
+  key-sharded run (src/pauli/propagate.h:157-264, 297-493).
+* The same circuit in FP32 and FP32 compensated: |d<O>|, relative coefficient
+  error on common keys and the key-set symmetric difference against FP64.
+* 512 uneven kicked-Ising instances (L 6..12, so up to 144 qubits and 3 key
+  words; depth 4..24) batched in every layout (one term store for all in
+  layer-group or window steps, one workgroup each), FP64 and FP32 compensated:
+  every instance's <O> and live term count against the oracle run per instance.
+* The 6x6 Fermi-Hubbard Majorana circuit (20 Trotter steps, 6960 rotations) in
+  both term orders, bit-identical to the serial Majorana oracle
+  (src/majorana_propagation.h:18-33); and the learned pruner on the 30-layer
+  kicked-Ising and the Fermi-Hubbard circuits within |d<O>| <= 1e-3 of the oracle.
 """
 from __future__ import annotations
 
@@ -31,7 +30,7 @@ from fastfermion_amd._native import Context, prune_opts
 pytestmark = pytest.mark.gpu
 
 THREADS = max(1, min(16, len(os.sched_getaffinity(0))))
-C2 = dict(maxdegree=8, mincoeff=1e-5)
+KI_OPTS = dict(maxdegree=8, mincoeff=1e-5)  # the 10x10 kicked-Ising truncation
 
 
 @pytest.fixture(scope="module")
@@ -51,7 +50,7 @@ def c2():
 @pytest.fixture(scope="module")
 def c2_oracle(c2):
     gates, x, z, c = c2
-    return {b: pyffo.pauli_propagate(gates, 2, x, z, c, batched=b, strategy=pyffo.SHARDED, n_threads=THREADS, **C2)
+    return {b: pyffo.pauli_propagate(gates, 2, x, z, c, batched=b, strategy=pyffo.SHARDED, n_threads=THREADS, **KI_OPTS)
             for b in (True, False)}
 
 
@@ -59,11 +58,11 @@ def c2_oracle(c2):
 def test_c2_full_size_bit_exact(ctx, c2, c2_oracle, batched):
     gates, x, z, c = c2
     ref = c2_oracle[batched]
-    dev = ctx.pauli_propagate(gates, 2, x, z, c, batched=batched, **C2)
+    dev = ctx.pauli_propagate(gates, 2, x, z, c, batched=batched, **KI_OPTS)
     p = ctx.profile()
-    print(f"C2 batched={batched}: {len(dev)} terms, {p['updates']} updates (oracle {ref.updates}), "
+    print(f"kicked Ising 10x10x20 batched={batched}: {len(dev)} terms, {p['updates']} updates (oracle {ref.updates}), "
           f"peak {p['peak_terms']}", flush=True)
-    assert_same_terms(dev, ref, exact=True, what=f"C2 batched={batched}")
+    assert_same_terms(dev, ref, exact=True, what=f"kicked Ising 10x10x20 batched={batched}")
     assert p["updates"] == ref.updates
     assert abs(ctx.expect_last([0, 0]) - pyffo.pauli_expect(ref, [0, 0])) < 1e-15
 
@@ -92,13 +91,13 @@ def test_c3_precision_errors(ctx, c2, c2_oracle, prec, max_d, max_rel, batched):
     distance (mincoeff 1e-5, float spacing 1e-12 there)."""
     gates, x, z, c = c2
     ref = c2_oracle[batched]
-    dev = ctx.pauli_propagate(gates, 2, x, z, c, batched=batched, precision=prec, **C2)
+    dev = ctx.pauli_propagate(gates, 2, x, z, c, batched=batched, precision=prec, **KI_OPTS)
     d, rmax, rmed, n_only, worst_only = precision_errors(dev, ref)
-    print(f"C3 prec={prec} batched={batched}: |d<O>| {d:.3g}, rel err max {rmax:.3g} median {rmed:.3g}, "
+    print(f"kicked Ising 10x10x20 prec={prec} batched={batched}: |d<O>| {d:.3g}, rel err max {rmax:.3g} median {rmed:.3g}, "
           f"{n_only} keys on one side (largest |c| {worst_only:.3g}) of {len(ref)}", flush=True)
     assert d <= max_d
     assert rmax <= max_rel
-    assert worst_only <= C2["mincoeff"] * (1 + 1e-5)
+    assert worst_only <= KI_OPTS["mincoeff"] * (1 + 1e-5)
 
 
 @pytest.fixture(scope="module")
@@ -144,7 +143,7 @@ def test_c4_all_512_instances(ctx, c4, c4_oracle, monkeypatch, prec, tol, layout
     want_e, want_n = c4_oracle
     d = np.abs(ex - want_e)
     bad_n = np.nonzero(nt != want_n)[0]
-    print(f"C4 {layout} prec={prec}: max |d<O>| {d.max():.3g}, term-count mismatches {len(bad_n)}, "
+    print(f"512-instance batch {layout} prec={prec}: max |d<O>| {d.max():.3g}, term-count mismatches {len(bad_n)}, "
           f"fallbacks {fb}", flush=True)
     assert d.max() <= tol, int(np.argmax(d))
     if prec == _abi.FP64:
@@ -163,7 +162,7 @@ def test_c5_fermi_hubbard_6x6_bit_exact(ctx, order):
     ref = pyffo.majorana_propagate(g, 3, m, c, maxdegree=8, mincoeff=1e-4)
     dev = ctx.majorana_propagate(g, 3, m, c, maxdegree=8, mincoeff=1e-4)
     p = ctx.profile()
-    print(f"C5 FH {order}: {len(dev)} terms, {p['updates']} updates (oracle {ref.updates}), "
+    print(f"Fermi-Hubbard 6x6 {order}: {len(dev)} terms, {p['updates']} updates (oracle {ref.updates}), "
           f"{p['run_launches']} launches", flush=True)
     assert_same_terms(dev, ref, exact=True, what=f"Fermi-Hubbard 6x6 {order}")
     assert p["updates"] == ref.updates
@@ -187,12 +186,12 @@ def test_c5_learned_pruner_kicked_ising_30_layers(ctx):
     F, T = pruner.pauli_logs(ctx, circ, 2, x, z, c, po, log_every=20, maxdegree=8, mincoeff=1e-5, batched=False)
     ctx.pruner_init(3)
     ctx.pruner_train(F, T, epochs=5, batch=4096, lr=1e-3, seed=1, under_weight=32.0)
-    ref = pyffo.pauli_propagate(g5, 2, x, z, c, batched=True, strategy=pyffo.SHARDED, n_threads=THREADS, **C2)
+    ref = pyffo.pauli_propagate(g5, 2, x, z, c, batched=True, strategy=pyffo.SHARDED, n_threads=THREADS, **KI_OPTS)
     want = pyffo.pauli_expect(ref, [0, 0])
     pl = prune_opts(row, col, obs_site, clifford_tol=1e-12, rot_per_layer=rpl, prune_period=rpl, log10_tau=-6.0)
     _, e = ctx.pauli_propagate_pruned(g5, 2, x, z, c, pl, maxdegree=8, mincoeff=1e-5, batched=False,
                                       precision=_abi.FP32C, download=False, want_expect=True)
-    print(f"C5 learned KI 10x10x30: |d<Z>| {abs(e - want):.3g} ({len(F)} log rows)", flush=True)
+    print(f"learned pruner, kicked Ising 10x10x30: |d<Z>| {abs(e - want):.3g} ({len(F)} log rows)", flush=True)
     assert abs(e - want) <= 1e-3
 
 
@@ -215,5 +214,5 @@ def test_c5_learned_pruner_fermi_hubbard(ctx):
     pf = prune_opts(frow, fcol, 0, rot_per_layer=frpl, prune_period=frpl, log10_tau=-7.0)
     _, e = ctx.majorana_propagate_pruned(fh, 3, fm, fc, pf, maxdegree=8, mincoeff=1e-4, precision=_abi.FP32C,
                                          download=False, occ=occw)
-    print(f"C5 learned FH 6x6: |d<n>| {abs(e - want):.3g} ({len(F)} log rows)", flush=True)
+    print(f"learned pruner, Fermi-Hubbard 6x6: |d<n>| {abs(e - want):.3g} ({len(F)} log rows)", flush=True)
     assert abs(e - want) <= 1e-3
