@@ -2848,8 +2848,7 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
         // cut by schedule length into cf.split term stores, each on its own stream: the
         // short instances' heavy steps then overlap the long ones' latency-bound tails
         // (512-instance kicked-Ising batch: one store 75 ms; cut at 60 % of the longest
-        // schedule 68 ms; at 45/70 %
-        // 69 ms; at 30/55/80 % 65 ms)
+        // schedule 68 ms; at 45 and 70 % 69 ms; at 30, 55 and 80 % 65 ms)
         static const std::vector<int> kCuts[4] = {{}, {60}, {45, 70}, {30, 55, 80}};
         const std::vector<int>& cuts = kCuts[cf.split - 1];
         const int np = (int)cuts.size() + 1;
