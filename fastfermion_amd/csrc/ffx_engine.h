@@ -1131,10 +1131,12 @@ struct Engine {
         int rot_grid = 2 * ctx->n_cu;
         if (const char* ev = getenv("FFX_ROT_GRID"))
             if (atoi(ev) > 0) rot_grid = atoi(ev);
-        // the layer kernels launch one resident round (4 blocks of 256 threads per CU): a
-        // round and a half left a tail of late blocks (kicked Ising per-gate 8.9 -> 8.5 ms,
-        // Fermi-Hubbard 6.1 -> 6.0 ms).  The window kernel keeps 6 per CU: on 3.7e7 live
-        // terms (maxdegree 16, no layers) 4 per CU took 1.63 s against 1.32 s.
+        // the disjoint / general layer kernel launches one resident round (4 blocks of 256
+        // threads per CU): a round and a half left a tail of late blocks (kicked Ising
+        // per-gate 8.9 -> 8.5 ms, Fermi-Hubbard 6.1 -> 6.0 ms).  The window and
+        // single-qubit layer kernels keep 6 per CU: on 3.7e7 live terms (maxdegree 16, no
+        // layers) 4 per CU took 1.63 s against 1.32 s, and the pruned step's RX layers
+        // (~3e3 terms, bigger tiles on fewer blocks) 39.5 us against 33.5 us.
         int win_grid = 6 * ctx->n_cu;
         int layer_grid = 4 * ctx->n_cu;
         if (const char* ev = getenv("FFX_WIN_GRID"))
@@ -1407,7 +1409,7 @@ struct Engine {
                         L.maxdeg = maxdeg;
                         L.max_xw = max_xw;
                         L.stamp = ctx->layer_stamp;
-                        hipLaunchKernelGGL((k_layer_async<Alg, CP, K>), dim3(layer_grid), dim3(kBlock), 0, ctx->stream,
+                        hipLaunchKernelGGL((k_layer_async<Alg, CP, K>), dim3(win_grid), dim3(kBlock), 0, ctx->stream,
                                            store(), (const store_t*)ctx->coef, (store_t*)ctx->coef2, L, ep, e);
                         CHK(launch_check());
                         if (timed) HIPCHK(hipEventRecord(ctx->kev[2 * nev++ + 1], ctx->stream));
