@@ -322,31 +322,45 @@ class Context:
     @staticmethod
     def _instances(instances, n_words):
         """ffx_instance records of a batch, filled column-wise in a numpy array of the
-        same layout (a ctypes field assignment per record costs ~10 us); returns the
-        record pointer and the arrays it points into, which must outlive the call."""
+        same layout (a ctypes field assignment per record costs ~10 us); the
+        observables of all instances are packed into one key / coefficient array each,
+        the gate tables stay where they are.  Returns the record pointer and the
+        arrays it points into, which must outlive the call."""
         n = len(instances)
         rec = np.zeros(max(n, 1), dtype=_INSTANCE_DTYPE)
-        keep = [rec]
-        cols = {k: [0] * n for k in ("gates", "n_gates", "x", "z", "c_re", "c_im", "n_terms")}
+        if n == 0:
+            return rec.ctypes.data_as(ctypes.POINTER(Instance)), [rec]
+        gl, xl, zl, cl = [], [], [], []
+        gptr = np.zeros(n, np.uint64)
+        ng = np.zeros(n, np.int64)
         for i, (gates, x, z, c, state) in enumerate(instances):
             gates = np.ascontiguousarray(gates, dtype=PAULI_GATE_DTYPE)
-            x = np.ascontiguousarray(x, dtype=np.uint64).reshape(-1, n_words)
-            z = np.ascontiguousarray(z, dtype=np.uint64).reshape(-1, n_words)
-            c = np.asarray(c, dtype=np.complex128).reshape(-1)
-            re = np.ascontiguousarray(c.real)
-            im = np.ascontiguousarray(c.imag) if c.imag.any() else None
-            keep += [gates, x, z, re, im]
-            cols["gates"][i], cols["n_gates"][i] = gates.ctypes.data, len(gates)
-            cols["x"][i], cols["z"][i], cols["c_re"][i] = x.ctypes.data, z.ctypes.data, re.ctypes.data
-            cols["c_im"][i] = 0 if im is None else im.ctypes.data
-            cols["n_terms"][i] = len(c)
+            gl.append(gates)
+            gptr[i] = gates.ctypes.data
+            ng[i] = len(gates)
+            xl.append(np.asarray(x, dtype=np.uint64).reshape(-1, n_words))
+            zl.append(np.asarray(z, dtype=np.uint64).reshape(-1, n_words))
+            cl.append(np.asarray(c, dtype=np.complex128).reshape(-1))
             if state:
                 for w, v in enumerate(list(state)[:4]):
                     rec["basis_state"][i, w] = int(v)
-        if n:
-            for k, v in cols.items():
-                rec[k][:n] = v
-        return rec.ctypes.data_as(ctypes.POINTER(Instance)), keep
+        X = np.ascontiguousarray(np.concatenate(xl))
+        Z = np.ascontiguousarray(np.concatenate(zl))
+        C = np.concatenate(cl)
+        RE = np.ascontiguousarray(C.real)
+        IM = np.ascontiguousarray(C.imag)
+        nt = np.array([len(c) for c in cl], np.int64)
+        toff = np.concatenate([[0], np.cumsum(nt)[:-1]]).astype(np.uint64)
+        nz = np.concatenate([[0], np.cumsum(IM != 0)])
+        has_im = (nz[toff.astype(np.int64) + nt] - nz[toff.astype(np.int64)]) > 0
+        rec["gates"][:n] = gptr
+        rec["n_gates"][:n] = ng
+        rec["x"][:n] = np.uint64(X.ctypes.data) + toff * np.uint64(8 * n_words)
+        rec["z"][:n] = np.uint64(Z.ctypes.data) + toff * np.uint64(8 * n_words)
+        rec["c_re"][:n] = np.uint64(RE.ctypes.data) + toff * np.uint64(8)
+        rec["c_im"][:n] = np.where(has_im, np.uint64(IM.ctypes.data) + toff * np.uint64(8), np.uint64(0))
+        rec["n_terms"][:n] = nt
+        return rec.ctypes.data_as(ctypes.POINTER(Instance)), [rec, X, Z, RE, IM] + gl
 
     def majorana_propagate(self, gates, n_words, m, c, maxdegree=None, mincoeff=0.0, precision=FP64,
                            download=True, occ=None, want_expect=False):
