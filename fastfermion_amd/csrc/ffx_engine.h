@@ -2836,6 +2836,10 @@ int run_pauli_batch(ffx_ctx* ctx, const ffx_instance* inst, int32_t n_inst, cons
     {
         std::vector<int32_t> all((size_t)n_inst);
         for (int32_t i = 0; i < n_inst; ++i) all[(size_t)i] = i;
+        // longest circuits first: the threads take instances in this order, so the
+        // big ones do not start last and leave the others idle
+        std::stable_sort(all.begin(), all.end(),
+                         [&](int32_t a, int32_t b) { return inst[a].n_gates > inst[b].n_gates; });
         parallel(all, build);
     }
     // largest estimated work first: workgroups are dispatched in launch order
