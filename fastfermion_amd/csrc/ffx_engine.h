@@ -2313,6 +2313,7 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
     // first group seen with each hash of (records, kind); a hit with equal bytes
     // takes that group's records instead of a new copy
     std::unordered_map<uint64_t, uint32_t> seen;
+    std::unordered_map<uint64_t, std::pair<size_t, size_t>> defseen;  // hash -> (first step, group)
     for (size_t t = 0; t < sd.size();) {
         // run on pairwise disjoint supports
         size_t kd = 0;
@@ -2361,6 +2362,44 @@ bool plan_mgroups(const std::vector<typename Engine<PauliAlg<W>, CP, K>::AStep>&
         }
         const bool general = !(kd >= 2 && kd >= kg);
         const size_t k = general ? std::max<size_t>(kg, 1) : kd;
+        // a group with the same rotations (axes, angles, truncation points) and kind as an
+        // earlier one (the same layer in another Trotter step) reuses its records as is
+        uint64_t dh = 0xC2B2AE3D27D4EB4Full ^ (uint64_t)k ^ ((uint64_t)general << 40);
+        auto mixin = [&](uint64_t v) {
+            dh = (dh ^ v) * 0xFF51AFD7ED558CCDull;
+            dh ^= dh >> 29;
+        };
+        for (size_t j = 0; j < k; ++j) {
+            const auto& a = sd[t + j];
+            for (int q = 0; q < NP; ++q) mixin(a.g.p[q]);
+            uint64_t b;
+            std::memcpy(&b, &a.cs, 8);
+            mixin(b);
+            std::memcpy(&b, &a.sn, 8);
+            mixin(b ^ ((uint64_t)a.trunc_thresh << 62) ^ ((uint64_t)a.trunc_xw << 61));
+        }
+        {
+            const auto it = defseen.find(dh);
+            if (it != defseen.end()) {
+                const size_t t0 = it->second.first;
+                const MGroup<W>& E = P.groups[it->second.second];
+                bool same = E.nr == (uint32_t)k && (E.general != 0) == general;
+                for (size_t j = 0; j < k && same; ++j) {
+                    const auto& a = sd[t + j];
+                    const auto& b = sd[t0 + j];
+                    for (int q = 0; q < NP; ++q) same = same && a.g.p[q] == b.g.p[q];
+                    same = same && std::memcmp(&a.cs, &b.cs, 8) == 0 && std::memcmp(&a.sn, &b.sn, 8) == 0 &&
+                           a.trunc_thresh == b.trunc_thresh && a.trunc_xw == b.trunc_xw;
+                }
+                if (same) {
+                    P.groups.push_back(E);
+                    t += k;
+                    continue;
+                }
+            } else {
+                defseen.emplace(dh, std::make_pair(t, P.groups.size()));
+            }
+        }
         MGroup<W> G{};
         G.gate_off = (uint32_t)P.gates.size();
         G.pos_off = (uint32_t)P.pos.size();
